@@ -1,0 +1,141 @@
+// Device context behind lfh_ctx: resident scene, render state, stream.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/lfh.h"
+#include "dev_common.h"
+#include "pt_types.h"
+
+namespace lfh {
+
+// RAII device buffer.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    hipError_t alloc(size_t n) {
+        release();
+        if (n == 0) n = 16;
+        hipError_t e = hipMalloc(&p, n);
+        if (e == hipSuccess) bytes = n;
+        else p = nullptr;
+        return e;
+    }
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// Host-side packed copy of a scene (built from an lfh_scene_desc).
+struct PackedScene {
+    std::vector<float> nodes;     // 16 floats / internal node
+    std::vector<float> tris;      // 12 floats / triangle slot
+    std::vector<float> insts;     // 16 floats / instance (inverse rows + meta)
+    std::vector<float> xforms;    // 16 floats / instance (forward, column-major)
+    std::vector<float> nmats;     // 12 floats / instance (normal matrix rows)
+    std::vector<int32_t> vidx;    // 4 int / triangle slot
+    std::vector<float> normals;   // 4 floats / vertex
+    std::vector<float> materials; // 28 floats / material
+    std::vector<float> lights;    // 28 floats / light (raw 16 + derived 12)
+    std::vector<float> hdr_cols;  // 4 floats / texel
+    std::vector<float> hdr_marg;  // 2 floats / row
+    std::vector<float> hdr_cond;  // 2 floats / texel
+    int hdr_w = 0, hdr_h = 0;
+    uint32_t root = lfhd::kRefSentinel;
+    int64_t num_internal = 0, num_tris = 0, num_vertices = 0;
+    int32_t num_instances = 0, num_materials = 0, num_lights = 0;
+    int max_depth_levels = 0;
+    // Node index maps for partial TLAS updates: flat index -> packed index.
+    std::vector<int32_t> flat_to_packed;
+    std::vector<int32_t> flat_lrleaf;   // cached LRLeaf (3 ints / flat node)
+    int32_t top_level_index = 0;
+};
+
+// Build the device layout from the reference flat arrays; "" on success.
+std::string pack_scene(const lfh_scene_desc& d, PackedScene* out);
+// Re-pack the instance-dependent parts after Renderer::Update-style edits.
+std::string repack_instances(PackedScene* ps, const float* transforms, const float* tlas_nodes,
+                             int64_t first_node, int64_t count, const float* materials);
+
+struct DeviceScene {
+    DevBuf nodes, tris, insts, xforms, nmats, vidx, normals, materials, lights, hdr_cols, hdr_marg, hdr_cond;
+    size_t bytes() const {
+        return nodes.bytes + tris.bytes + insts.bytes + xforms.bytes + nmats.bytes + vidx.bytes + normals.bytes +
+               materials.bytes + lights.bytes + hdr_cols.bytes + hdr_marg.bytes + hdr_cond.bytes;
+    }
+};
+
+// Accumulation buffers for the whole frame.
+struct RenderState {
+    DevBuf accum, comp;   // float4 per pixel: running sum, Kahan compensation
+    int npaths = 0;       // tile pixels
+    int64_t npix = 0;
+};
+
+// Path state + queues of one wavefront pass for up to `cap` paths.
+struct PassBufs {
+    DevBuf rad, thr, absorb, rng, misc, nee;  // per path (nee: 3 float4)
+    DevBuf rq0, rq1, hq, sq;                  // ray queues, hit queue, shadow queue
+    DevBuf counters, work;
+    int cap = 0;
+    hipError_t alloc(int n) {
+        hipError_t e;
+        const size_t f4 = 16;
+        DevBuf* b[10] = {&rad, &thr, &absorb, &rng, &misc, &nee, &rq0, &rq1, &hq, &sq};
+        const size_t m[10] = {1, 1, 1, 1, 1, 3, 2, 2, 2, 4};
+        for (int i = 0; i < 10; ++i)
+            if ((e = b[i]->alloc(f4 * m[i] * (size_t)n)) != hipSuccess) return e;
+        if ((e = counters.alloc(64)) != hipSuccess) return e;
+        if ((e = work.alloc(64)) != hipSuccess) return e;
+        cap = n;
+        return hipSuccess;
+    }
+    size_t bytes() const {
+        return rad.bytes + thr.bytes + absorb.bytes + rng.bytes + misc.bytes + nee.bytes + rq0.bytes + rq1.bytes +
+               hq.bytes + sq.bytes + counters.bytes + work.bytes;
+    }
+};
+
+hipError_t upload_scene(const PackedScene& ps, DeviceScene* ds, hipStream_t s);
+lfhd::TraceScene trace_view(const PackedScene& ps, const DeviceScene& ds);
+lfhd::ShadeScene shade_view(const PackedScene& ps, const DeviceScene& ds, const lfh_options& o);
+
+}  // namespace lfh
+
+struct lfh_ctx {
+    std::string err;
+    lfh_options opt{};
+    lfh::PackedScene ps;
+    lfh::DeviceScene ds;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    // traversal scratch
+    lfh::DevBuf spill, work, errflag;
+    int grid_blocks = 0;
+    float cull = 1.0f + 1.0f / 1024.0f;
+    // host staging for the host-array entry points
+    lfh::DevBuf stage_in, stage_out;
+    // render state
+    lfh::RenderState rs;
+    lfh::PassBufs pass;
+    lfh::DevBuf out;   // HDR output staging
+    int pt_grid = 0;
+    lfh_camera cam{};
+    bool has_cam = false;
+    // TiledRenderer counters (TiledRenderer.cpp:48-66, 471-501)
+    int tileX = -1, tileY = 0, numTilesX = 1, numTilesY = 1, tileW = 0, tileH = 0;
+    int sampleCounter = 1, frameCounter = 1;
+    // guiding
+    struct lfh_guide* guide = nullptr;
+    int guide_train_every = 0;
+};

@@ -1,0 +1,674 @@
+// Wavefront path tracer for gfx950.
+//
+// One TiledRenderer pass over a tile (renderer.glsl:main + PathTrace,
+// pathtrace.glsl:208-295) is split into stages that pass SoA queues through
+// HBM:
+//   generate  camera rays for the tile pixels, path state init
+//   extend    closest hit (analytic lights first, then the BVH) per queued ray
+//   shade     miss / emitter / surface shading, next-event candidates, BSDF
+//             sampling, Russian roulette; appends continuing rays and shadow
+//             rays with one atomic per wave (ballot + rank)
+//   shadow    any hit per shadow ray; clears the candidate when occluded
+//   accumulate  pending next-event terms + Kahan-compensated accumulation
+// The random-number stream of each path is the reference's: it lives in the
+// path state and is consumed in the reference's order; next-event terms are
+// evaluated before their shadow test (they consume no random numbers) and
+// added after it, in the reference's summation order.
+#include "dev_shade.h"
+#include "pt_kernels.h"
+
+namespace lfhd {
+
+// ---- queue helpers ---------------------------------------------------------------
+// Wave-aggregated append: returns this lane's slot (valid only if `want`).
+LFH_DEV uint32_t wave_append(uint32_t* counter, bool want) {
+    const uint64_t mask = __ballot(want);
+    if (mask == 0) return 0;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)mask) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+}
+
+LFH_DEV float4 f4(V3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
+
+__global__ void k_pt_reset(uint32_t* counters, unsigned long long* work) {
+    int i = threadIdx.x;
+    if (i < 8) counters[i] = 0u;
+    if (i < 8) work[i] = 0ull;
+}
+
+__global__ void k_pt_advance(uint32_t* consumed, uint32_t* shadow_count, unsigned long long* work) {
+    if (threadIdx.x == 0) {
+        *consumed = 0u;
+        *shadow_count = 0u;
+    }
+    if (threadIdx.x < 8) work[threadIdx.x] = 0ull;
+}
+
+// generate: one thread per tile pixel
+__global__ __launch_bounds__(256) void k_generate(CamParams cam, PathBufs pb, float4* __restrict__ rayq,
+                                                  uint32_t* __restrict__ count) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int P = cam.tile_w * cam.tile_h;
+    bool valid = false;
+    int lx = 0, ly = 0;
+    if (p < P) {
+        lx = p % cam.tile_w;
+        ly = p / cam.tile_w;
+        valid = (cam.tileX * cam.tile_w + lx < cam.width) && (cam.tileY * cam.tile_h + ly < cam.height);
+    }
+    V3 o = v3(0.f, 0.f, 0.f), d = v3(0.f, 0.f, 1.f);
+    uint4 seed = make_uint4(0, 0, 0, 0);
+    if (valid) {
+        camera_ray(cam, lx, ly, o, d, seed);
+        pb.rad[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+        pb.thr[p] = make_float4(1.f, 1.f, 1.f, 0.f);
+        pb.absorb[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+        pb.rng[p] = seed;
+        pb.misc[p] = make_float4(0.f, __int_as_float(0), 0.f, 0.f);
+        pb.nee[p * 3 + 2] = make_float4(0.f, 0.f, 0.f, 0.f);
+    } else if (p < P) {
+        pb.rad[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+        pb.nee[p * 3 + 2] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    uint32_t slot = wave_append(count, valid);
+    if (valid) {
+        rayq[(size_t)slot * 2] = f4(o, __int_as_float(p));
+        rayq[(size_t)slot * 2 + 1] = f4(d, 0.f);
+    }
+}
+
+// generate from caller-given rays and seeds (lfh_path_trace_rays)
+__global__ __launch_bounds__(256) void k_generate_rays(const float* __restrict__ rays6, const uint4* __restrict__ seeds,
+                                                       int n, PathBufs pb, float4* __restrict__ rayq,
+                                                       uint32_t* __restrict__ count) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    bool valid = p < n;
+    if (valid) {
+        pb.rad[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+        pb.thr[p] = make_float4(1.f, 1.f, 1.f, 0.f);
+        pb.absorb[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+        pb.rng[p] = seeds[p];
+        pb.misc[p] = make_float4(0.f, __int_as_float(0), 0.f, 0.f);
+        pb.nee[p * 3 + 2] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    uint32_t slot = wave_append(count, valid);
+    if (valid) {
+        const float* r = rays6 + (size_t)p * 6;
+        rayq[(size_t)slot * 2] = make_float4(r[0], r[1], r[2], __int_as_float(p));
+        rayq[(size_t)slot * 2 + 1] = make_float4(r[3], r[4], r[5], 0.f);
+    }
+}
+
+// ---- extend: closest hit, lights first (closest_hit.glsl:7-67) ---------------------
+LFH_DEV void light_closest(const ShadeScene& sc, V3 o, V3 d, float& t, int& light, float& lpdf) {
+    for (int i = 0; i < sc.num_lights; ++i) {
+        const float4* lp = sc.lights + (size_t)i * kLightF4;
+        float4 a = lp[0], b = lp[1], c = lp[2];
+        float type = a.w, area = b.w, radius = c.w;
+        if (type == 0.0f) {
+            V3 normal = ld3(lp[4]);
+            if (dot(normal, d) > 0.0f) continue;
+            float dd = rect_hit(lp, o, d);
+            if (dd < 0.0f) dd = kInfinity;
+            if (dd < t) {
+                t = dd;
+                float cosTheta = dot(-d, normal);
+                lpdf = (t * t) / (area * cosTheta);
+                light = i;
+            }
+        }
+        if (type == 1.0f) {
+            float dd = sphere_hit(radius, ld3(a), o, d);
+            if (dd < 0.0f) dd = kInfinity;
+            if (dd < t) {
+                t = dd;
+                lpdf = (t * t) / area;
+                light = i;
+            }
+        }
+    }
+}
+
+LFH_DEV bool light_any(const ShadeScene& sc, V3 o, V3 d, float maxDist) {
+    for (int i = 0; i < sc.num_lights; ++i) {
+        const float4* lp = sc.lights + (size_t)i * kLightF4;
+        float4 a = lp[0], c = lp[2];
+        if (a.w == 0.0f) {
+            float dd = rect_hit(lp, o, d);
+            if (dd > 0.0f && dd < maxDist) return true;
+        }
+        if (a.w == 1.0f) {
+            float dd = sphere_hit(c.w, ld3(a), o, d);
+            if (dd > 0.0f && dd < maxDist) return true;
+        }
+    }
+    return false;
+}
+
+template <bool SHADOW>
+__global__ __launch_bounds__(kTraceBlock) void k_trace_queue(ShadeScene sc, const float4* __restrict__ q,
+                                                             const uint32_t* __restrict__ count,
+                                                             float4* __restrict__ hitq, PathBufs pb,
+                                                             uint32_t* __restrict__ spill,
+                                                             unsigned long long* __restrict__ work, float cull,
+                                                             int* __restrict__ err) {
+    __shared__ uint32_t stk[kTraceBlock / 64][kLdsStack][64];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const size_t gtid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    LaneStack st;
+    st.lds = &stk[wave][0][lane];
+    st.spill = spill + gtid * kSpillStride;
+    st.reset();
+    const int64_t n = (int64_t)*count;
+    TraceState s;
+    bool active = false;
+    int64_t rid = 0;
+    int light = -1;
+    float lpdf = 0.f;
+    for (;;) {
+        const bool need = !active;
+        const uint64_t mask = __ballot(need);
+        if (mask) {
+            const int leader = __ffsll((long long)mask) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(work, (unsigned long long)__popcll(mask));
+            base = __shfl(base, leader);
+            if (need) {
+                rid = (int64_t)base + __popcll(mask & ((1ull << lane) - 1ull));
+                if (rid < n) {
+                    float4 a = q[rid * 2], b = q[rid * 2 + 1];
+                    V3 o = v3(a.x, a.y, a.z), d = v3(b.x, b.y, b.z);
+                    if (SHADOW) {
+                        float maxd = b.w;
+                        trace_init(s, st, o, d, 0.0f, maxd, sc.tr.root);
+                        if (light_any(sc, o, d, maxd)) {
+                            s.occluded = true;
+                            s.cur = kRefSentinel;  // finishes at once
+                        }
+                    } else {
+                        float t = kInfinity;
+                        light = -1;
+                        lpdf = 0.f;
+                        light_closest(sc, o, d, t, light, lpdf);
+                        trace_init(s, st, o, d, 0.0f, t, sc.tr.root);
+                    }
+                    active = true;
+                }
+            }
+        }
+        if (__ballot(active) == 0) break;
+        if (active) {
+            bool done = trace_run<SHADOW, false>(sc.tr, s, st, cull, 48, nullptr);
+            if (done) {
+                if (st.overflow) atomicOr(err, 1);
+                if (SHADOW) {
+                    if (s.occluded) {
+                        float4 a = q[rid * 2];
+                        uint32_t tag = __float_as_uint(a.w);
+                        uint32_t p = tag >> 1, kind = tag & 1u;
+                        pb.nee[(size_t)p * 3 + kind].w = 0.0f;
+                    }
+                } else {
+                    float4 h0, h1;
+                    if (s.bestPrim >= 0) {
+                        h0 = make_float4(s.tbest, s.bu, s.bv, __int_as_float(s.bestSlot));
+                        h1 = make_float4(__int_as_float(s.bestInst), __int_as_float(s.bestMat), 0.f, __int_as_float(-1));
+                    } else {
+                        h0 = make_float4(s.tbest, 0.f, 0.f, __int_as_float(-1));
+                        h1 = make_float4(__int_as_float(-1), __int_as_float(-1), lpdf, __int_as_float(light));
+                    }
+                    hitq[rid * 2] = h0;
+                    hitq[rid * 2 + 1] = h1;
+                }
+                active = false;
+            }
+        }
+    }
+}
+
+// ---- shade ------------------------------------------------------------------------
+LFH_DEV void combine_nee(PathBufs& pb, int p, V3& rad) {
+    float4 tn = pb.nee[(size_t)p * 3 + 2];
+    if (tn.w != 0.0f) {
+        float4 e = pb.nee[(size_t)p * 3 + 0], l = pb.nee[(size_t)p * 3 + 1];
+        V3 Li = v3(0.f, 0.f, 0.f);
+        if (e.w != 0.0f) Li = Li + ld3(e);
+        if (l.w != 0.0f) Li = Li + ld3(l);
+        rad = rad + Li * ld3(tn);
+        pb.nee[(size_t)p * 3 + 2] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+}
+
+LFH_DEV void onb(V3 N, V3& T, V3& B) {
+    V3 up = fabsf(N.z) < 0.999f ? v3(0.f, 0.f, 1.f) : v3(1.f, 0.f, 0.f);
+    T = normalize(cross(up, N));
+    B = cross(N, T);
+}
+
+__global__ __launch_bounds__(256) void k_shade(ShadeScene sc, int depth, const float4* __restrict__ rayq,
+                                               const float4* __restrict__ hitq, const uint32_t* __restrict__ count,
+                                               float4* __restrict__ nextq, uint32_t* __restrict__ next_count,
+                                               float4* __restrict__ shadowq, uint32_t* __restrict__ shadow_count,
+                                               PathBufs pb, GuideHook gh) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = *count;
+    const bool live = i < n;
+    bool cont = false;
+    bool shEnv = false, shLight = false;
+    V3 nextO = v3(0.f, 0.f, 0.f), nextD = v3(0.f, 0.f, 0.f);
+    V3 shO = v3(0.f, 0.f, 0.f), shEnvD = v3(0.f, 0.f, 0.f), shLightD = v3(0.f, 0.f, 0.f);
+    float shLightMax = 0.f;
+    int p = 0;
+    if (live) {
+        float4 r0 = rayq[(size_t)i * 2], r1 = rayq[(size_t)i * 2 + 1];
+        p = __float_as_int(r0.w);
+        V3 ro = ld3(r0), rd = ld3(r1);
+        float4 h0 = hitq[(size_t)i * 2], h1 = hitq[(size_t)i * 2 + 1];
+        V3 rad = ld3(pb.rad[p]);
+        V3 thr = ld3(pb.thr[p]);
+        combine_nee(pb, p, rad);
+        float4 misc = pb.misc[p];
+        float prevPdf = misc.x;
+        int lastMat = __float_as_int(misc.y);
+        const float t = h0.x;
+        const int light = __float_as_int(h1.w);
+        const int slot = __float_as_int(h0.w);
+        if (t == kInfinity) {
+            // miss (pathtrace.glsl:223-244)
+            if (sc.const_bg) {
+                rad = rad + v3(sc.bg[0], sc.bg[1], sc.bg[2]) * thr;
+            } else if (sc.use_env) {
+                float misWeight = 1.0f;
+                float u = (kPi + atan2f(rd.z, rd.x)) * (1.0f / kTwoPi);
+                float v = acosf(rd.y) * (1.0f / kPi);
+                if (depth > 0) {
+                    float lightPdf = EnvPdf(sc, rd);
+                    misWeight = powerHeuristic(prevPdf, lightPdf);
+                }
+                rad = rad + misWeight * tex_linear(sc.hdr_cols, sc.hdr_w, sc.hdr_h, u, v) * thr * sc.hdr_mult;
+            }
+        } else if (slot < 0 && light >= 0) {
+            // analytic light is the closest hit (pathtrace.glsl:246-260): the
+            // material fetched is the last surface's (state is not refreshed)
+            Mat m = load_material(sc.materials, lastMat);
+            rad = rad + m.emission * thr;
+            V3 em = ld3(sc.lights[(size_t)light * kLightF4 + 1]);
+            V3 Le = (depth == 0) ? em : powerHeuristic(prevPdf, h1.z) * em;
+            rad = rad + Le * thr;
+        } else {
+            const int inst = __float_as_int(h1.x);
+            const int matId = __float_as_int(h1.y);
+            float u = h0.y, v = h0.z;
+            float w = 1.0f - u - v;
+            // hit point: object-space ray point, then the instance transform
+            // (closest_hit.glsl:140-143)
+            const float4* ip = sc.tr.insts + (size_t)inst * 4;
+            float4 i0 = ip[0], i1 = ip[1], i2 = ip[2];
+            V3 oo = xform_inv(i0, i1, i2, ro, 1.0f);
+            V3 od = xform_inv(i0, i1, i2, rd, 0.0f);
+            V3 po = oo + od * t;
+            const float4* xp = sc.xforms + (size_t)inst * 4;
+            float4 c0 = xp[0], c1 = xp[1], c2 = xp[2], c3 = xp[3];
+            V3 fhp = v3(c0.x * po.x + c1.x * po.y + c2.x * po.z + c3.x * 1.0f,
+                        c0.y * po.x + c1.y * po.y + c2.y * po.z + c3.y * 1.0f,
+                        c0.z * po.x + c1.z * po.y + c2.z * po.z + c3.z * 1.0f);
+            // shading normal (pathtrace.glsl:16-37)
+            int4 vi = sc.vidx[slot];
+            V3 n1 = ld3(sc.normals[vi.x]), n2 = ld3(sc.normals[vi.y]), n3 = ld3(sc.normals[vi.z]);
+            V3 nrm = normalize(n1 * w + n2 * u + n3 * v);
+            const float4* np = sc.nmats + (size_t)inst * 3;
+            float4 m0 = np[0], m1 = np[1], m2 = np[2];
+            nrm = normalize(v3(m0.x * nrm.x + m0.y * nrm.y + m0.z * nrm.z, m1.x * nrm.x + m1.y * nrm.y + m1.z * nrm.z,
+                               m2.x * nrm.x + m2.y * nrm.y + m2.z * nrm.z));
+            Frame fr;
+            fr.N = dot(nrm, rd) <= 0.0f ? nrm : nrm * -1.0f;
+            onb(nrm, fr.T, fr.B);
+            Mat m = load_material(sc.materials, matId);
+            fr.eta = dot(rd, nrm) < 0.0f ? (1.0f / m.ior) : m.ior;
+            V3 absorb = ld3(pb.absorb[p]);
+            if (dot(nrm, fr.N) > 0.0f) absorb = v3(0.f, 0.f, 0.f);
+            rad = rad + m.emission * thr;
+            lastMat = matId;
+            thr = thr * v3(expf(-absorb.x * t), expf(-absorb.y * t), expf(-absorb.z * t));
+            uint4 rng = pb.rng[p];
+            // next-event estimation (pathtrace.glsl:126-204)
+            const V3 V = -rd;
+            V3 surfacePos = fhp + nrm * kEps;
+            shO = surfacePos;
+            float4 e4 = make_float4(0.f, 0.f, 0.f, 0.f), l4 = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (sc.use_env) {
+                V3 color, ldir;
+                float lightPdf;
+                EnvSample(sc, rng, color, ldir, lightPdf);
+                float bp;
+                V3 f = DisneyEval(m, fr.eta, V, fr.N, ldir, bp);
+                if (bp > 0.0f) {
+                    float mw = powerHeuristic(lightPdf, bp);
+                    if (mw > 0.0f) {
+                        V3 c = mw * f * fabsf(dot(ldir, fr.N)) * color / lightPdf;
+                        e4 = f4(c, 1.0f);
+                        shEnv = true;
+                        shEnvD = ldir;
+                    }
+                }
+            }
+            if (sc.num_lights > 0) {
+                int index = (int)(rnd(rng) * (float)sc.num_lights);
+                if (index >= sc.num_lights) index = sc.num_lights - 1;
+                LightRec lr;
+                sampleOneLight(sc, index, surfacePos, lr, rng);
+                if (dot(lr.direction, lr.normal) < 0.0f) {
+                    float bp;
+                    V3 f = DisneyEval(m, fr.eta, V, fr.N, lr.direction, bp);
+                    float area = sc.lights[(size_t)index * kLightF4 + 1].w;
+                    float weight = 1.0f;
+                    if (area > 0.0f) weight = powerHeuristic(lr.pdf, bp);
+                    if (bp > 0.0f) {
+                        V3 c = weight * f * fabsf(dot(fr.N, lr.direction)) * lr.emission / lr.pdf;
+                        l4 = f4(c, 1.0f);
+                        shLight = true;
+                        shLightD = lr.direction;
+                        shLightMax = lr.dist - kEps;
+                    }
+                }
+            }
+            if (shEnv || shLight) {
+                pb.nee[(size_t)p * 3 + 0] = e4;
+                pb.nee[(size_t)p * 3 + 1] = l4;
+                pb.nee[(size_t)p * 3 + 2] = f4(thr, 1.0f);
+            }
+            // BSDF sampling (pathtrace.glsl:268-291)
+            V3 L;
+            float pdf;
+            V3 f = DisneySample(m, fr, V, L, pdf, rng);
+            if (dot(fr.N, L) < 0.0f)
+                absorb = v3(-logf(m.extinction.x) / m.atDistance, -logf(m.extinction.y) / m.atDistance,
+                            -logf(m.extinction.z) / m.atDistance);
+            bool alive = pdf > 0.0f;
+            if (alive) thr = thr * ((f * fabsf(dot(fr.N, L))) / pdf);
+            if (alive && sc.rr && depth >= sc.rr_depth) {
+                float q = fminf(fmaxf(thr.x, fmaxf(thr.y, thr.z)) + 0.001f, 0.95f);
+                if (rnd(rng) > q) alive = false;
+                else thr = thr / q;
+            }
+            if (gh.enabled) guide_record(gh, p, depth, fhp, nrm, V, L, pdf, thr, alive);
+            if (alive && depth + 1 < sc.max_depth) {
+                cont = true;
+                nextD = L;
+                nextO = fhp + L * kEps;
+            }
+            pb.rng[p] = rng;
+            pb.thr[p] = f4(thr, 0.f);
+            pb.absorb[p] = f4(absorb, 0.f);
+            prevPdf = pdf;
+        }
+        pb.rad[p] = f4(rad, 0.f);
+        pb.misc[p] = make_float4(prevPdf, __int_as_float(lastMat), 0.f, 0.f);
+    }
+    uint32_t s = wave_append(next_count, cont);
+    if (cont) {
+        nextq[(size_t)s * 2] = f4(nextO, __int_as_float(p));
+        nextq[(size_t)s * 2 + 1] = f4(nextD, 0.f);
+    }
+    uint32_t se = wave_append(shadow_count, shEnv);
+    if (shEnv) {
+        shadowq[(size_t)se * 2] = f4(shO, __uint_as_float(((uint32_t)p << 1) | 0u));
+        shadowq[(size_t)se * 2 + 1] = f4(shEnvD, kInfinity - kEps);
+    }
+    uint32_t sl = wave_append(shadow_count, shLight);
+    if (shLight) {
+        shadowq[(size_t)sl * 2] = f4(shO, __uint_as_float(((uint32_t)p << 1) | 1u));
+        shadowq[(size_t)sl * 2 + 1] = f4(shLightD, shLightMax);
+    }
+}
+
+// ---- accumulate: pending NEE + Kahan sum into the frame -----------------------------
+__global__ __launch_bounds__(256) void k_accumulate(PathBufs pb, int P, int tile_w, int tileX, int tileY, int tile_h,
+                                                    int width, int height, float4* __restrict__ accum,
+                                                    float4* __restrict__ comp, int kahan) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    const int lx = p % tile_w, ly = p / tile_w;
+    const int gx = tileX * tile_w + lx, gy = tileY * tile_h + ly;
+    if (gx >= width || gy >= height) return;
+    V3 rad = ld3(pb.rad[p]);
+    combine_nee(pb, p, rad);
+    const size_t pix = (size_t)gy * width + gx;
+    float4 s = accum[pix];
+    if (kahan) {
+        float4 c = comp[pix];
+        float yx = rad.x - c.x, yy = rad.y - c.y, yz = rad.z - c.z;
+        float tx = s.x + yx, ty = s.y + yy, tz = s.z + yz;
+        c.x = (tx - s.x) - yx;
+        c.y = (ty - s.y) - yy;
+        c.z = (tz - s.z) - yz;
+        comp[pix] = c;
+        accum[pix] = make_float4(tx, ty, tz, s.w + 1.0f);
+    } else {
+        accum[pix] = make_float4(s.x + rad.x, s.y + rad.y, s.z + rad.z, s.w + 1.0f);
+    }
+}
+
+// radiance out for lfh_path_trace_rays
+__global__ void k_collect(PathBufs pb, int n, float* __restrict__ out) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    V3 rad = ld3(pb.rad[p]);
+    combine_nee(pb, p, rad);
+    out[(size_t)p * 3] = rad.x;
+    out[(size_t)p * 3 + 1] = rad.y;
+    out[(size_t)p * 3 + 2] = rad.z;
+}
+
+// linear HDR output: (sum - compensation) / samples, bottom-up rows
+__global__ void k_output(const float4* __restrict__ accum, const float4* __restrict__ comp, int64_t npix, float inv,
+                         float* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npix) return;
+    float4 s = accum[i], c = comp[i];
+    out[i * 3] = (s.x - c.x) * inv;
+    out[i * 3 + 1] = (s.y - c.y) * inv;
+    out[i * 3 + 2] = (s.z - c.z) * inv;
+}
+
+__global__ void k_clear4(float4* a, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// ---- per-stage parity kernels ---------------------------------------------------------
+LFH_DEV void load_frame(const float* fr, Frame& f, V3& V, int& mat) {
+    f.N = v3(fr[0], fr[1], fr[2]);
+    f.T = v3(fr[3], fr[4], fr[5]);
+    f.B = v3(fr[6], fr[7], fr[8]);
+    V = v3(fr[9], fr[10], fr[11]);
+    f.eta = fr[12];
+    mat = (int)fr[13];
+}
+
+__global__ void k_eval_bsdf(ShadeScene sc, const float* frames, const float* L, int n, float* f, float* pdf) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Frame fr;
+    V3 V;
+    int mat;
+    load_frame(frames + (size_t)i * 16, fr, V, mat);
+    Mat m = load_material(sc.materials, mat);
+    float p;
+    V3 r = DisneyEval(m, fr.eta, V, fr.N, v3(L[i * 3], L[i * 3 + 1], L[i * 3 + 2]), p);
+    f[i * 3] = r.x;
+    f[i * 3 + 1] = r.y;
+    f[i * 3 + 2] = r.z;
+    pdf[i] = p;
+}
+
+__global__ void k_sample_bsdf(ShadeScene sc, const float* frames, uint4* seeds, int n, float* Lo, float* f,
+                              float* pdf) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Frame fr;
+    V3 V;
+    int mat;
+    load_frame(frames + (size_t)i * 16, fr, V, mat);
+    Mat m = load_material(sc.materials, mat);
+    uint4 s = seeds[i];
+    V3 L = v3(0.f, 0.f, 0.f);
+    float p;
+    V3 r = DisneySample(m, fr, V, L, p, s);
+    seeds[i] = s;
+    Lo[i * 3] = L.x;
+    Lo[i * 3 + 1] = L.y;
+    Lo[i * 3 + 2] = L.z;
+    f[i * 3] = r.x;
+    f[i * 3 + 1] = r.y;
+    f[i * 3 + 2] = r.z;
+    pdf[i] = p;
+}
+
+__global__ void k_sample_light(ShadeScene sc, const float* pos, uint4* seeds, int n, float* dir, float* nrm,
+                               float* em, float* dist, float* pdf) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint4 s = seeds[i];
+    int index = (int)(rnd(s) * (float)sc.num_lights);
+    if (index >= sc.num_lights) index = sc.num_lights - 1;
+    LightRec r;
+    sampleOneLight(sc, index, v3(pos[i * 3], pos[i * 3 + 1], pos[i * 3 + 2]), r, s);
+    seeds[i] = s;
+    dir[i * 3] = r.direction.x; dir[i * 3 + 1] = r.direction.y; dir[i * 3 + 2] = r.direction.z;
+    nrm[i * 3] = r.normal.x; nrm[i * 3 + 1] = r.normal.y; nrm[i * 3 + 2] = r.normal.z;
+    em[i * 3] = r.emission.x; em[i * 3 + 1] = r.emission.y; em[i * 3 + 2] = r.emission.z;
+    dist[i] = r.dist;
+    pdf[i] = r.pdf;
+}
+
+__global__ void k_sample_env(ShadeScene sc, uint4* seeds, int n, float* dir, float* pdf, float* col) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint4 s = seeds[i];
+    V3 c, d;
+    float p;
+    EnvSample(sc, s, c, d, p);
+    seeds[i] = s;
+    dir[i * 3] = d.x; dir[i * 3 + 1] = d.y; dir[i * 3 + 2] = d.z;
+    col[i * 3] = c.x; col[i * 3 + 1] = c.y; col[i * 3 + 2] = c.z;
+    pdf[i] = p;
+}
+
+__global__ void k_env_pdf(ShadeScene sc, const float* dir, int n, float* pdf) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    pdf[i] = EnvPdf(sc, v3(dir[i * 3], dir[i * 3 + 1], dir[i * 3 + 2]));
+}
+
+}  // namespace lfhd
+
+// ===========================================================================
+// host launchers
+// ===========================================================================
+using namespace lfhd;
+
+static inline int nblk(int64_t n, int b = 256) { return (int)((n + b - 1) / b); }
+
+hipError_t pt_reset(uint32_t* counters, unsigned long long* work, hipStream_t s) {
+    hipLaunchKernelGGL(k_pt_reset, dim3(1), dim3(64), 0, s, counters, work);
+    return hipGetLastError();
+}
+
+hipError_t pt_generate(const CamParams& cam, const PathBufs& pb, float4* rayq, uint32_t* count, hipStream_t s) {
+    int P = cam.tile_w * cam.tile_h;
+    hipLaunchKernelGGL(k_generate, dim3(nblk(P)), dim3(256), 0, s, cam, pb, rayq, count);
+    return hipGetLastError();
+}
+
+hipError_t pt_generate_rays(const float* rays6, const uint4* seeds, int n, const PathBufs& pb, float4* rayq,
+                            uint32_t* count, hipStream_t s) {
+    hipLaunchKernelGGL(k_generate_rays, dim3(nblk(n)), dim3(256), 0, s, rays6, seeds, n, pb, rayq, count);
+    return hipGetLastError();
+}
+
+hipError_t pt_bounce(const ShadeScene& sc, int depth, int maxq, float4* rayq, uint32_t* count, float4* nextq,
+                     uint32_t* next_count, float4* hitq, float4* shadowq, uint32_t* shadow_count, const PathBufs& pb,
+                     uint32_t* spill, unsigned long long* work, int* err, float cull, int grid_blocks,
+                     const GuideHook& gh, hipStream_t s) {
+    hipLaunchKernelGGL((k_trace_queue<false>), dim3(grid_blocks), dim3(kTraceBlock), 0, s, sc, rayq, count, hitq, pb,
+                       spill, work, cull, err);
+    hipLaunchKernelGGL(k_shade, dim3(nblk(maxq)), dim3(256), 0, s, sc, depth, rayq, hitq, count, nextq, next_count,
+                       shadowq, shadow_count, pb, gh);
+    hipLaunchKernelGGL((k_trace_queue<true>), dim3(grid_blocks), dim3(kTraceBlock), 0, s, sc, shadowq, shadow_count,
+                       hitq, pb, spill, work + 1, cull, err);
+    hipLaunchKernelGGL(k_pt_advance, dim3(1), dim3(64), 0, s, count, shadow_count, work);
+    return hipGetLastError();
+}
+
+hipError_t pt_accumulate(const PathBufs& pb, const CamParams& cam, float4* accum, float4* comp, int kahan,
+                         hipStream_t s) {
+    int P = cam.tile_w * cam.tile_h;
+    hipLaunchKernelGGL(k_accumulate, dim3(nblk(P)), dim3(256), 0, s, pb, P, cam.tile_w, cam.tileX, cam.tileY,
+                       cam.tile_h, cam.width, cam.height, accum, comp, kahan);
+    return hipGetLastError();
+}
+
+hipError_t pt_collect(const PathBufs& pb, int n, float* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_collect, dim3(nblk(n)), dim3(256), 0, s, pb, n, out);
+    return hipGetLastError();
+}
+
+hipError_t pt_output(const float4* accum, const float4* comp, int64_t npix, float inv, float* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_output, dim3(nblk(npix)), dim3(256), 0, s, accum, comp, npix, inv, out);
+    return hipGetLastError();
+}
+
+hipError_t pt_clear4(float4* a, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_clear4, dim3(nblk(n)), dim3(256), 0, s, a, n);
+    return hipGetLastError();
+}
+
+int pt_trace_grid_blocks() {
+    static int blocks = 0;
+    if (blocks == 0) {
+        int dev = 0, a = 0, b = 0;
+        hipGetDevice(&dev);
+        hipDeviceProp_t prop;
+        hipGetDeviceProperties(&prop, dev);
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, (const void*)k_trace_queue<false>, kTraceBlock, 0) !=
+                hipSuccess || a <= 0)
+            a = 4;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, (const void*)k_trace_queue<true>, kTraceBlock, 0) !=
+                hipSuccess || b <= 0)
+            b = 4;
+        blocks = (a > b ? a : b) * prop.multiProcessorCount;
+    }
+    return blocks;
+}
+
+hipError_t stage_eval_bsdf(const ShadeScene& sc, const float* fr, const float* L, int n, float* f, float* pdf,
+                           hipStream_t s) {
+    hipLaunchKernelGGL(k_eval_bsdf, dim3(nblk(n)), dim3(256), 0, s, sc, fr, L, n, f, pdf);
+    return hipGetLastError();
+}
+hipError_t stage_sample_bsdf(const ShadeScene& sc, const float* fr, uint4* seeds, int n, float* L, float* f,
+                             float* pdf, hipStream_t s) {
+    hipLaunchKernelGGL(k_sample_bsdf, dim3(nblk(n)), dim3(256), 0, s, sc, fr, seeds, n, L, f, pdf);
+    return hipGetLastError();
+}
+hipError_t stage_sample_light(const ShadeScene& sc, const float* pos, uint4* seeds, int n, float* dir, float* nrm,
+                              float* em, float* dist, float* pdf, hipStream_t s) {
+    hipLaunchKernelGGL(k_sample_light, dim3(nblk(n)), dim3(256), 0, s, sc, pos, seeds, n, dir, nrm, em, dist, pdf);
+    return hipGetLastError();
+}
+hipError_t stage_sample_env(const ShadeScene& sc, uint4* seeds, int n, float* dir, float* pdf, float* col,
+                            hipStream_t s) {
+    hipLaunchKernelGGL(k_sample_env, dim3(nblk(n)), dim3(256), 0, s, sc, seeds, n, dir, pdf, col);
+    return hipGetLastError();
+}
+hipError_t stage_env_pdf(const ShadeScene& sc, const float* dir, int n, float* pdf, hipStream_t s) {
+    hipLaunchKernelGGL(k_env_pdf, dim3(nblk(n)), dim3(256), 0, s, sc, dir, n, pdf);
+    return hipGetLastError();
+}

@@ -1,0 +1,399 @@
+// Reference flat layout -> device layout (Renderer::Init's upload step,
+// Renderer.cpp:76-188, re-designed for gfx950 loads).
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ctx.h"
+#include "xform.h"
+
+namespace lfh {
+
+using lfhd::kRefSentinel;
+
+static inline float u2f(uint32_t u) {
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+static void put_light(const float* l, float* out) {
+    // raw: pos(3) emission(3) u(3) v(3) radius area type
+    const float* p = l;
+    const float* e = l + 3;
+    const float* u = l + 6;
+    const float* v = l + 9;
+    float radius = l[12], area = l[13], type = l[14];
+    float o[28] = {p[0], p[1], p[2], type, e[0], e[1], e[2], area, u[0], u[1], u[2], radius, v[0], v[1], v[2], 0.f};
+    // derived quad-light quantities (closest_hit.glsl:30-35)
+    float cx = u[1] * v[2] - u[2] * v[1];
+    float cy = u[2] * v[0] - u[0] * v[2];
+    float cz = u[0] * v[1] - u[1] * v[0];
+    float len = std::sqrt(cx * cx + cy * cy + cz * cz);
+    float nx = cx / len, ny = cy / len, nz = cz / len;
+    float pw = nx * p[0] + ny * p[1] + nz * p[2];
+    float iu = 1.0f / (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    float iv = 1.0f / (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    float d[12] = {nx, ny, nz, pw, u[0] * iu, u[1] * iu, u[2] * iu, 0.f, v[0] * iv, v[1] * iv, v[2] * iv, 0.f};
+    std::memcpy(out, o, 16 * sizeof(float));
+    std::memcpy(out + 16, d, 12 * sizeof(float));
+}
+
+static void put_instance(PackedScene* ps, int i, const float* t16, uint32_t blasRef, int mat) {
+    float m[4][4];
+    std::memcpy(m, t16, sizeof m);
+    float inv[4][4];
+    inverse4(m, inv);
+    float* r = &ps->insts[(size_t)i * 16];
+    for (int row = 0; row < 3; ++row) {
+        r[row * 4 + 0] = inv[0][row];
+        r[row * 4 + 1] = inv[1][row];
+        r[row * 4 + 2] = inv[2][row];
+        r[row * 4 + 3] = inv[3][row];
+    }
+    r[12] = u2f(blasRef);
+    r[13] = u2f((uint32_t)mat);
+    r[14] = u2f((uint32_t)i);
+    r[15] = 0.f;
+    std::memcpy(&ps->xforms[(size_t)i * 16], t16, 16 * sizeof(float));
+    float nm[3][3];
+    normal_matrix(m, nm);
+    float* n = &ps->nmats[(size_t)i * 12];
+    for (int row = 0; row < 3; ++row) {
+        n[row * 4 + 0] = nm[row][0];
+        n[row * 4 + 1] = nm[row][1];
+        n[row * 4 + 2] = nm[row][2];
+        n[row * 4 + 3] = 0.f;
+    }
+}
+
+namespace {
+struct Builder {
+    const lfh_scene_desc& d;
+    PackedScene* ps;
+    std::vector<int32_t> lr;  // 3 per flat node
+    std::vector<uint32_t> inst_blas;
+    std::vector<int32_t> inst_mat;
+    std::string err;
+
+    explicit Builder(const lfh_scene_desc& dd, PackedScene* p) : d(dd), ps(p) {}
+
+    bool ref_of(int64_t n, uint32_t* out) {
+        if (n < 0 || n >= d.num_nodes) { err = "node index out of range"; return false; }
+        int32_t leaf = lr[n * 3 + 2];
+        if (leaf > 0) {
+            int32_t first = lr[n * 3], count = lr[n * 3 + 1];
+            if (count < 1 || count > 7) { err = "BLAS leaf with " + std::to_string(count) + " triangles (1..7 supported)"; return false; }
+            if (first < 0 || (int64_t)first + count > d.num_tris || first >= (1 << 28)) { err = "BLAS leaf triangle range out of bounds"; return false; }
+            *out = lfhd::make_leaf((uint32_t)first, (uint32_t)count);
+        } else if (leaf < 0) {
+            int64_t inst = -(int64_t)leaf - 1;
+            if (inst >= d.num_instances) { err = "TLAS leaf instance out of range"; return false; }
+            *out = lfhd::make_inst((uint32_t)inst);
+        } else {
+            int32_t p = ps->flat_to_packed[n];
+            if (p < 0) { err = "internal node not reachable"; return false; }
+            *out = (uint32_t)p;
+        }
+        return true;
+    }
+
+    bool run() {
+        const int64_t N = d.num_nodes;
+        if (!d.nodes || N <= 0) { err = "no BVH nodes"; return false; }
+        if (d.top_level_index < 0 || d.top_level_index >= N) { err = "top_level_index out of range"; return false; }
+        if (d.num_tris > 0 && (!d.vert_indices || !d.vertices_uvx)) { err = "missing triangle arrays"; return false; }
+        if (d.num_instances > 0 && !d.transforms) { err = "missing transforms"; return false; }
+        if (d.num_materials <= 0 || !d.materials) { err = "at least one material is required"; return false; }
+        lr.resize(N * 3);
+        for (int64_t i = 0; i < N; ++i)
+            for (int k = 0; k < 3; ++k) lr[i * 3 + k] = (int32_t)d.nodes[i * 9 + 6 + k];
+        // Reachable internal nodes (DFS from the top-level node, entering
+        // BLAS roots through TLAS leaves).
+        std::vector<uint8_t> seen(N, 0);
+        std::vector<std::pair<int64_t, int>> stack{{d.top_level_index, 0}};
+        inst_blas.assign(d.num_instances, kRefSentinel);
+        inst_mat.assign(d.num_instances, 0);
+        int maxlev = 0;
+        while (!stack.empty()) {
+            auto [n, lev] = stack.back();
+            stack.pop_back();
+            if (n < 0 || n >= N) { err = "child index out of range"; return false; }
+            if (seen[n]) continue;
+            seen[n] = 1;
+            maxlev = std::max(maxlev, lev);
+            int32_t leaf = lr[n * 3 + 2];
+            if (leaf == 0) {
+                stack.push_back({lr[n * 3 + 0], lev + 1});
+                stack.push_back({lr[n * 3 + 1], lev + 1});
+            } else if (leaf < 0) {
+                int64_t inst = -(int64_t)leaf - 1;
+                if (inst >= d.num_instances) { err = "TLAS leaf instance out of range"; return false; }
+                stack.push_back({lr[n * 3 + 0], lev + 1});
+            }
+        }
+        // Acyclic by construction: every internal node's children come after
+        // it (preorder flattening, bvh_translator.cpp:35-89), and no TLAS leaf
+        // may be reached again below a BLAS root.  A desc violating either
+        // could make the traversal loop, so it is rejected here.
+        std::vector<uint8_t> inBlas(N, 0);
+        for (int64_t i = 0; i < N; ++i) {
+            if (!seen[i]) continue;
+            int32_t leaf = lr[i * 3 + 2];
+            if (leaf == 0) {
+                if (lr[i * 3] <= i || lr[i * 3 + 1] <= i) { err = "BVH child index not after its parent"; return false; }
+            } else if (leaf < 0) {
+                int64_t b = lr[i * 3];
+                if (b < 0 || b >= N) { err = "BLAS root out of range"; return false; }
+                inBlas[b] = 1;
+            }
+        }
+        for (int64_t i = 0; i < N; ++i) {  // propagate "below a BLAS root" in index order
+            if (!seen[i] || !inBlas[i]) continue;
+            int32_t leaf = lr[i * 3 + 2];
+            if (leaf < 0) { err = "nested instancing (TLAS leaf inside a BLAS) is not supported"; return false; }
+            if (leaf == 0) {
+                inBlas[lr[i * 3]] = 1;
+                inBlas[lr[i * 3 + 1]] = 1;
+            }
+        }
+        ps->max_depth_levels = maxlev;
+        if (maxlev + 4 > (int)96) { err = "BVH deeper than the traversal stack (" + std::to_string(maxlev) + " levels)"; return false; }
+        ps->flat_to_packed.assign(N, -1);
+        int64_t np = 0;
+        for (int64_t i = 0; i < N; ++i)
+            if (seen[i] && lr[i * 3 + 2] == 0) ps->flat_to_packed[i] = (int32_t)np++;
+        if (np >= (1 << 30)) { err = "too many BVH nodes"; return false; }
+        ps->num_internal = np;
+        ps->nodes.assign((size_t)np * 16, 0.f);
+        for (int64_t i = 0; i < N; ++i) {
+            if (!(seen[i] && lr[i * 3 + 2] == 0)) continue;
+            int64_t p = ps->flat_to_packed[i];
+            int64_t L = lr[i * 3], R = lr[i * 3 + 1];
+            uint32_t rl, rr;
+            if (!ref_of(L, &rl) || !ref_of(R, &rr)) return false;
+            const float* a = &d.nodes[L * 9];
+            const float* b = &d.nodes[R * 9];
+            float* o = &ps->nodes[p * 16];
+            o[0] = a[0]; o[1] = a[1]; o[2] = a[2]; o[3] = a[3];
+            o[4] = a[4]; o[5] = a[5]; o[6] = b[0]; o[7] = b[1];
+            o[8] = b[2]; o[9] = b[3]; o[10] = b[4]; o[11] = b[5];
+            o[12] = u2f(rl); o[13] = u2f(rr); o[14] = 0.f; o[15] = 0.f;
+        }
+        for (int64_t i = 0; i < N; ++i) {
+            if (!seen[i] || lr[i * 3 + 2] >= 0) continue;
+            int64_t inst = -(int64_t)lr[i * 3 + 2] - 1;
+            uint32_t br;
+            if (!ref_of(lr[i * 3], &br)) return false;
+            if (lfhd::ref_is_inst(br)) { err = "TLAS leaf points at another TLAS leaf"; return false; }
+            inst_blas[inst] = br;
+            inst_mat[inst] = lr[i * 3 + 1];
+        }
+        if (!ref_of(d.top_level_index, &ps->root)) return false;
+        ps->top_level_index = d.top_level_index;
+        ps->flat_lrleaf = lr;
+
+        // triangles
+        ps->num_tris = d.num_tris;
+        ps->num_vertices = d.num_vertices;
+        ps->tris.assign((size_t)d.num_tris * 12, 0.f);
+        ps->vidx.assign((size_t)d.num_tris * 4, 0);
+        for (int64_t k = 0; k < d.num_tris; ++k) {
+            int32_t vi[3] = {d.vert_indices[k * 3], d.vert_indices[k * 3 + 1], d.vert_indices[k * 3 + 2]};
+            for (int j = 0; j < 3; ++j)
+                if (vi[j] < 0 || vi[j] >= d.num_vertices) { err = "vertex index out of range"; return false; }
+            const float* v0 = &d.vertices_uvx[(size_t)vi[0] * 4];
+            const float* v1 = &d.vertices_uvx[(size_t)vi[1] * 4];
+            const float* v2 = &d.vertices_uvx[(size_t)vi[2] * 4];
+            float* t = &ps->tris[k * 12];
+            t[0] = v0[0]; t[1] = v0[1]; t[2] = v0[2]; t[3] = u2f((uint32_t)(vi[0] / 3));
+            t[4] = v1[0] - v0[0]; t[5] = v1[1] - v0[1]; t[6] = v1[2] - v0[2]; t[7] = u2f((uint32_t)k);
+            t[8] = v2[0] - v0[0]; t[9] = v2[1] - v0[1]; t[10] = v2[2] - v0[2]; t[11] = 0.f;
+            ps->vidx[k * 4 + 0] = vi[0];
+            ps->vidx[k * 4 + 1] = vi[1];
+            ps->vidx[k * 4 + 2] = vi[2];
+        }
+        ps->normals.assign((size_t)d.num_vertices * 4, 0.f);
+        if (d.normals_uvy && d.num_vertices > 0)
+            std::memcpy(ps->normals.data(), d.normals_uvy, (size_t)d.num_vertices * 16);
+
+        // instances
+        ps->num_instances = d.num_instances;
+        ps->insts.assign((size_t)d.num_instances * 16, 0.f);
+        ps->xforms.assign((size_t)d.num_instances * 16, 0.f);
+        ps->nmats.assign((size_t)d.num_instances * 12, 0.f);
+        for (int i = 0; i < d.num_instances; ++i) {
+            if (inst_mat[i] < 0 || inst_mat[i] >= d.num_materials) { err = "instance material out of range"; return false; }
+            put_instance(ps, i, &d.transforms[(size_t)i * 16], inst_blas[i], inst_mat[i]);
+        }
+        // materials
+        ps->num_materials = d.num_materials;
+        ps->materials.assign(d.materials, d.materials + (size_t)d.num_materials * 28);
+        // lights
+        ps->num_lights = d.num_lights;
+        ps->lights.assign((size_t)d.num_lights * 28, 0.f);
+        for (int i = 0; i < d.num_lights; ++i) put_light(&d.lights[(size_t)i * 15], &ps->lights[(size_t)i * 28]);
+        // environment
+        ps->hdr_w = d.hdr_width;
+        ps->hdr_h = d.hdr_height;
+        if (d.hdr_width > 0 && d.hdr_height > 0) {
+            if (!d.hdr_cols || !d.hdr_marginal || !d.hdr_conditional) { err = "incomplete HDR arrays"; return false; }
+            size_t n = (size_t)d.hdr_width * d.hdr_height;
+            ps->hdr_cols.assign(n * 4, 0.f);
+            for (size_t i = 0; i < n; ++i)
+                for (int c = 0; c < 3; ++c) ps->hdr_cols[i * 4 + c] = d.hdr_cols[i * 3 + c];
+            ps->hdr_marg.assign(d.hdr_marginal, d.hdr_marginal + (size_t)d.hdr_height * 2);
+            ps->hdr_cond.assign(d.hdr_conditional, d.hdr_conditional + n * 2);
+        }
+        return true;
+    }
+};
+}  // namespace
+
+std::string pack_scene(const lfh_scene_desc& d, PackedScene* out) {
+    Builder b(d, out);
+    if (!b.run()) return b.err.empty() ? std::string("invalid scene") : b.err;
+    return std::string();
+}
+
+// Renderer::Update re-uploads transforms, materials and the TLAS node range
+// (Renderer.cpp:192-203); here the TLAS nodes are re-packed from the new flat
+// nodes, which must keep the BLAS part and the packed node count unchanged.
+std::string repack_instances(PackedScene* ps, const float* transforms, const float* tlas_nodes, int64_t first_node,
+                             int64_t count, const float* materials) {
+    if (materials) ps->materials.assign(materials, materials + (size_t)ps->num_materials * 28);
+    std::vector<uint32_t> blas(ps->num_instances);
+    std::vector<int32_t> mat(ps->num_instances);
+    for (int i = 0; i < ps->num_instances; ++i) {
+        uint32_t b;
+        std::memcpy(&b, &ps->insts[(size_t)i * 16 + 12], 4);
+        blas[i] = b;
+        uint32_t m;
+        std::memcpy(&m, &ps->insts[(size_t)i * 16 + 13], 4);
+        mat[i] = (int32_t)m;
+    }
+    if (tlas_nodes && count > 0) {
+        const int64_t N = (int64_t)ps->flat_to_packed.size();
+        if (first_node < 0 || first_node + count > N) return "TLAS node range out of bounds";
+        // Updated LRLeaf + boxes of the flat nodes in range.
+        std::vector<float> flat((size_t)N * 9, 0.f);
+        // Only the in-range nodes are needed; rebuild packed TLAS nodes.
+        for (int64_t i = 0; i < count; ++i)
+            for (int k = 0; k < 3; ++k) ps->flat_lrleaf[(first_node + i) * 3 + k] = (int32_t)tlas_nodes[i * 9 + 6 + k];
+        for (int64_t i = 0; i < count; ++i) {
+            int64_t n = first_node + i;
+            int32_t leaf = ps->flat_lrleaf[n * 3 + 2];
+            if (leaf < 0) {
+                int64_t inst = -(int64_t)leaf - 1;
+                if (inst >= ps->num_instances) return "TLAS leaf instance out of range";
+                mat[inst] = ps->flat_lrleaf[n * 3 + 1];
+            }
+        }
+        auto box_of = [&](int64_t c, float* dst) -> bool {
+            if (c >= first_node && c < first_node + count) {
+                std::memcpy(dst, &tlas_nodes[(c - first_node) * 9], 6 * sizeof(float));
+                return true;
+            }
+            return false;
+        };
+        for (int64_t i = 0; i < count; ++i) {
+            int64_t n = first_node + i;
+            if (ps->flat_lrleaf[n * 3 + 2] != 0) continue;
+            int32_t p = ps->flat_to_packed[n];
+            if (p < 0) return "TLAS update changes the node structure";
+            int64_t L = ps->flat_lrleaf[n * 3], R = ps->flat_lrleaf[n * 3 + 1];
+            float a[6], b[6];
+            if (!box_of(L, a) || !box_of(R, b)) return "TLAS update references nodes outside the range";
+            auto ref = [&](int64_t c, uint32_t* r) -> bool {
+                int32_t lf = ps->flat_lrleaf[c * 3 + 2];
+                if (lf < 0) { *r = lfhd::make_inst((uint32_t)(-(int64_t)lf - 1)); return true; }
+                if (lf > 0) return false;
+                if (ps->flat_to_packed[c] < 0) return false;
+                *r = (uint32_t)ps->flat_to_packed[c];
+                return true;
+            };
+            uint32_t rl, rr;
+            if (!ref(L, &rl) || !ref(R, &rr)) return "TLAS update changes the node structure";
+            float* o = &ps->nodes[(size_t)p * 16];
+            o[0] = a[0]; o[1] = a[1]; o[2] = a[2]; o[3] = a[3];
+            o[4] = a[4]; o[5] = a[5]; o[6] = b[0]; o[7] = b[1];
+            o[8] = b[2]; o[9] = b[3]; o[10] = b[4]; o[11] = b[5];
+            o[12] = u2f(rl); o[13] = u2f(rr);
+        }
+    }
+    if (transforms)
+        for (int i = 0; i < ps->num_instances; ++i) put_instance(ps, i, &transforms[(size_t)i * 16], blas[i], mat[i]);
+    else
+        for (int i = 0; i < ps->num_instances; ++i) {
+            uint32_t m = (uint32_t)mat[i];
+            std::memcpy(&ps->insts[(size_t)i * 16 + 13], &m, 4);
+        }
+    return std::string();
+}
+
+template <class T>
+static hipError_t up(DevBuf& b, const std::vector<T>& v, hipStream_t s) {
+    hipError_t e = b.alloc(v.size() * sizeof(T));
+    if (e != hipSuccess) return e;
+    if (!v.empty()) return hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s);
+    return hipSuccess;
+}
+
+hipError_t upload_scene(const PackedScene& ps, DeviceScene* ds, hipStream_t s) {
+    hipError_t e;
+    if ((e = up(ds->nodes, ps.nodes, s)) != hipSuccess) return e;
+    if ((e = up(ds->tris, ps.tris, s)) != hipSuccess) return e;
+    if ((e = up(ds->insts, ps.insts, s)) != hipSuccess) return e;
+    if ((e = up(ds->xforms, ps.xforms, s)) != hipSuccess) return e;
+    if ((e = up(ds->nmats, ps.nmats, s)) != hipSuccess) return e;
+    if ((e = up(ds->vidx, ps.vidx, s)) != hipSuccess) return e;
+    if ((e = up(ds->normals, ps.normals, s)) != hipSuccess) return e;
+    if ((e = up(ds->materials, ps.materials, s)) != hipSuccess) return e;
+    if ((e = up(ds->lights, ps.lights, s)) != hipSuccess) return e;
+    if ((e = up(ds->hdr_cols, ps.hdr_cols, s)) != hipSuccess) return e;
+    if ((e = up(ds->hdr_marg, ps.hdr_marg, s)) != hipSuccess) return e;
+    if ((e = up(ds->hdr_cond, ps.hdr_cond, s)) != hipSuccess) return e;
+    return hipStreamSynchronize(s);
+}
+
+lfhd::TraceScene trace_view(const PackedScene& ps, const DeviceScene& ds) {
+    lfhd::TraceScene t;
+    t.nodes = ds.nodes.as<const float4>();
+    t.tris = ds.tris.as<const float4>();
+    t.insts = ds.insts.as<const float4>();
+    t.root = ps.root;
+    t.spill_stride = 96;
+    return t;
+}
+
+lfhd::ShadeScene shade_view(const PackedScene& ps, const DeviceScene& ds, const lfh_options& o) {
+    lfhd::ShadeScene s;
+    s.tr = trace_view(ps, ds);
+    s.xforms = ds.xforms.as<const float4>();
+    s.nmats = ds.nmats.as<const float4>();
+    s.vidx = ds.vidx.as<const int4>();
+    s.normals = ds.normals.as<const float4>();
+    s.materials = ds.materials.as<const float4>();
+    s.lights = ds.lights.as<const float4>();
+    s.hdr_cols = ds.hdr_cols.as<const float4>();
+    s.hdr_marg = ds.hdr_marg.as<const float2>();
+    s.hdr_cond = ds.hdr_cond.as<const float2>();
+    s.hdr_w = ps.hdr_w;
+    s.hdr_h = ps.hdr_h;
+    s.hdr_res = (float)(ps.hdr_w * ps.hdr_h);
+    s.hdr_mult = o.hdr_multiplier;
+    s.num_lights = ps.num_lights;
+    s.max_depth = o.max_depth;
+    s.rr = o.enable_rr;
+    s.rr_depth = o.rr_depth;
+    bool env = o.use_env_map && ps.hdr_w > 0;
+    s.use_env = (env && !o.use_constant_bg) ? 1 : 0;
+    s.const_bg = o.use_constant_bg ? 1 : 0;
+    s.bg[0] = o.bg_color[0];
+    s.bg[1] = o.bg_color[1];
+    s.bg[2] = o.bg_color[2];
+    return s;
+}
+
+}  // namespace lfh

@@ -1,0 +1,9 @@
+// Path-tracing renderer behind lfh_update / lfh_render (TiledRenderer
+// semantics) and the per-stage parity entry points.
+#pragma once
+#include "ctx.h"
+
+namespace lfh {
+int render_init(lfh_ctx* c);
+void render_release(lfh_ctx* c);
+}  // namespace lfh
