@@ -1,0 +1,197 @@
+// lfh_renderer.hpp — C++ drop-in for LavaFrame's Renderer / TiledRenderer.
+//
+// HipRendererT<SceneT> has the method set and semantics of the abstract
+// Renderer (LavaFrame/Renderer.h:71-117) as implemented by TiledRenderer
+// (LavaFrame/TiledRenderer.h:57-71), and reaches HIP only through the C ABI
+// of lfh.h.  SceneT is any type with the reference Scene's public members
+// (LavaFrame/Scene.h:67-101): the reference's LavaFrame::Scene itself, or
+// this build's lfh::Scene (lavaframe_amd/csrc/host_scene.h).
+//
+//   Init()                 Renderer::Init + TiledRenderer::Init: uploads the
+//                          flattened scene (Renderer.cpp:76-188)
+//   Update(secondsElapsed) TiledRenderer::Update: re-uploads instances when
+//                          scene->instancesModified, resets on camera motion,
+//                          else advances tile / sample / frame counters
+//   Render()               TiledRenderer::Render: one tile, one sample/pixel
+//   GetSampleCount(), GetProgress()
+//   GetOutputBufferHDR()   linear accum / samples, RGB float, bottom-up rows;
+//                          allocates with new[] like the reference (the caller
+//                          frees with delete[])
+//   GetOutputBuffer()      RGB8 of the same linear image, clamped (no tonemap;
+//                          tonemapping is display-only and not part of the core)
+//   Present(), SetViewport(), Denoise()  headless no-ops
+#pragma once
+
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "lfh.h"
+
+namespace lfh {
+
+template <class SceneT>
+class HipRendererT {
+public:
+    HipRendererT(SceneT* scene, const std::string& shadersDirectory = std::string())
+        : scene_(scene), shadersDirectory_(shadersDirectory) {}
+    virtual ~HipRendererT() { Finish(); }
+    HipRendererT(const HipRendererT&) = delete;
+    HipRendererT& operator=(const HipRendererT&) = delete;
+
+    virtual void Init() {
+        if (ctx_) return;
+        if (!scene_) throw std::runtime_error("HipRenderer: no scene");
+        lfh_scene_desc d = describe();
+        lfh_options o = options();
+        int r = lfh_create(&d, &o, &ctx_);
+        if (r != LFH_OK) throw std::runtime_error(std::string("HipRenderer::Init: ") + lfh_last_error(nullptr));
+        width_ = o.width;
+        height_ = o.height;
+    }
+
+    virtual void Finish() {
+        if (ctx_) lfh_destroy(ctx_);
+        ctx_ = nullptr;
+    }
+
+    virtual void Update(float /*secondsElapsed*/) {
+        if (!ctx_) return;
+        bool reset = scene_->camera->isMoving || scene_->instancesModified;
+        if (scene_->instancesModified) {
+            // Renderer::Update: transforms, materials and the TLAS node range
+            const int64_t top = scene_->bvhTranslator.topLevelIndex;
+            const int64_t count = (int64_t)scene_->bvhTranslator.nodes.size() - top;
+            check(lfh_update_instances(ctx_, reinterpret_cast<const float*>(scene_->transforms.data()),
+                                       reinterpret_cast<const float*>(scene_->materials.data()),
+                                       reinterpret_cast<const float*>(&scene_->bvhTranslator.nodes[top]), top, count),
+                  "Update");
+            scene_->instancesModified = false;  // the reference clears it in Render
+        }
+        lfh_camera c = camera();
+        check(lfh_update(ctx_, &c, reset ? 1 : 0), "Update");
+    }
+
+    virtual void Render() {
+        if (!ctx_) return;
+        check(lfh_render(ctx_), "Render");
+    }
+
+    virtual void Present() const {}
+
+    virtual float GetProgress() const {
+        float p = 0.f;
+        if (ctx_) lfh_get_progress(ctx_, &p);
+        return p;
+    }
+
+    virtual int GetSampleCount() const {
+        int32_t n = 0;
+        if (ctx_) lfh_get_sample_count(ctx_, &n);
+        return n;
+    }
+
+    virtual void GetOutputBufferHDR(float** data, int& w, int& h) {
+        w = width_;
+        h = height_;
+        *data = new float[(size_t)w * h * 3];
+        check(lfh_get_output_hdr(ctx_, *data, w, h), "GetOutputBufferHDR");
+    }
+
+    virtual void GetOutputBuffer(unsigned char** data, int& w, int& h) {
+        float* hdr = nullptr;
+        GetOutputBufferHDR(&hdr, w, h);
+        *data = new unsigned char[(size_t)w * h * 3];
+        for (size_t i = 0; i < (size_t)w * h * 3; ++i) {
+            float v = hdr[i] < 0.f ? 0.f : (hdr[i] > 1.f ? 1.f : hdr[i]);
+            (*data)[i] = (unsigned char)(v * 255.f + 0.5f);
+        }
+        delete[] hdr;
+    }
+
+    virtual uint32_t SetViewport(int, int) { return 0; }
+    virtual uint32_t Denoise() { return 0; }
+
+    lfh_ctx* context() const { return ctx_; }
+
+protected:
+    lfh_scene_desc describe() const {
+        const SceneT& s = *scene_;
+        lfh_scene_desc d;
+        std::memset(&d, 0, sizeof d);
+        d.nodes = reinterpret_cast<const float*>(s.bvhTranslator.nodes.data());
+        d.num_nodes = (int64_t)s.bvhTranslator.nodes.size();
+        d.top_level_index = s.bvhTranslator.topLevelIndex;
+        d.vert_indices = reinterpret_cast<const int32_t*>(s.vertIndices.data());
+        d.num_tris = (int64_t)s.vertIndices.size();
+        d.vertices_uvx = reinterpret_cast<const float*>(s.verticesUVX.data());
+        d.normals_uvy = reinterpret_cast<const float*>(s.normalsUVY.data());
+        d.num_vertices = (int64_t)s.verticesUVX.size();
+        d.materials = reinterpret_cast<const float*>(s.materials.data());
+        d.num_materials = (int32_t)s.materials.size();
+        d.transforms = reinterpret_cast<const float*>(s.transforms.data());
+        d.num_instances = (int32_t)s.transforms.size();
+        d.lights = reinterpret_cast<const float*>(s.lights.data());
+        d.num_lights = (int32_t)s.lights.size();
+        if (s.hdrData && s.renderOptions.useEnvMap) {
+            d.hdr_width = s.hdrData->width;
+            d.hdr_height = s.hdrData->height;
+            d.hdr_cols = hdr_cols(*s.hdrData);
+            d.hdr_marginal = reinterpret_cast<const float*>(hdr_ptr(s.hdrData->marginalDistData));
+            d.hdr_conditional = reinterpret_cast<const float*>(hdr_ptr(s.hdrData->conditionalDistData));
+        }
+        return d;
+    }
+
+    lfh_options options() const {
+        const auto& r = scene_->renderOptions;
+        lfh_options o;
+        std::memset(&o, 0, sizeof o);
+        o.width = r.resolution.x;
+        o.height = r.resolution.y;
+        o.tile_width = r.tileWidth;
+        o.tile_height = r.tileHeight;
+        o.max_depth = r.maxDepth;
+        o.enable_rr = r.enableRR ? 1 : 0;
+        o.rr_depth = r.RRDepth;
+        o.use_env_map = (r.useEnvMap && scene_->hdrData) ? 1 : 0;
+        o.use_constant_bg = r.useConstantBg ? 1 : 0;
+        o.bg_color[0] = r.bgColor.x;
+        o.bg_color[1] = r.bgColor.y;
+        o.bg_color[2] = r.bgColor.z;
+        o.hdr_multiplier = r.hdrMultiplier;
+        o.kahan = 1;
+        return o;
+    }
+
+    lfh_camera camera() const {
+        const auto& k = *scene_->camera;
+        lfh_camera c;
+        c.position[0] = k.position.x; c.position[1] = k.position.y; c.position[2] = k.position.z;
+        c.right[0] = k.right.x; c.right[1] = k.right.y; c.right[2] = k.right.z;
+        c.up[0] = k.up.x; c.up[1] = k.up.y; c.up[2] = k.up.z;
+        c.forward[0] = k.forward.x; c.forward[1] = k.forward.y; c.forward[2] = k.forward.z;
+        c.fov = k.fov;
+        c.focal_dist = k.focalDist;
+        c.aperture = k.aperture;
+        return c;
+    }
+
+    void check(int r, const char* what) const {
+        if (r != LFH_OK) throw std::runtime_error(std::string("HipRenderer::") + what + ": " + lfh_last_error(ctx_));
+    }
+
+    // HDR data is a raw array in the reference (float* / Vec2*) and a vector here.
+    template <class T> static const T* hdr_ptr(const T* p) { return p; }
+    template <class V> static const typename V::value_type* hdr_ptr(const V& v) { return v.data(); }
+    template <class H> static const float* hdr_cols(const H& h) { return hdr_ptr(h.cols); }
+
+    SceneT* scene_;
+    std::string shadersDirectory_;
+    lfh_ctx* ctx_ = nullptr;
+    int width_ = 0, height_ = 0;
+};
+
+}  // namespace lfh

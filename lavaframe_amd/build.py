@@ -20,6 +20,8 @@ OBJ_DIR = PKG / "_lib" / "obj"
 LIB = OUT_DIR / "liblfh.so"
 ARCH = os.environ.get("LFH_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+HOSTCXX = os.environ.get("LFH_HOSTCXX", "/opt/rocm/bin/amdclang++")
+HOST = ["-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include"]
 
 COMMON = ["-std=c++17", "-O3", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall",
           "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-but-set-variable",
@@ -33,7 +35,7 @@ def _compile(src: Path) -> Path:
     if src.suffix == ".hip":
         cmd = [HIPCC, *COMMON, *DEVICE, "-x", "hip", "-c", str(src), "-o", str(obj)]
     else:
-        cmd = [HIPCC, *COMMON, "-D__HIP_PLATFORM_AMD__", "-c", str(src), "-o", str(obj)]
+        cmd = [HOSTCXX, *COMMON, *HOST, "-c", str(src), "-o", str(obj)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {src.name}\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
@@ -48,13 +50,21 @@ def build(verbose: bool = True) -> Path:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs),
-           "-Wl,--no-undefined", "-lpthread"]
+           "-Wl,--no-undefined", "-L/opt/rocm/lib", "-lamdhip64", "-lpthread"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    # headless CLI over the C ABI (links the library just built)
+    cli = PKG / "cli" / "lfh_render.cpp"
+    exe = OUT_DIR / "lfh_render"
+    cmd = [HOSTCXX, *COMMON, *HOST, str(cli), "-o", str(exe), f"-L{OUT_DIR}", "-llfh",
+           "-Wl,-rpath,$ORIGIN", "-x", "none", str(OBJ_DIR / "host_scene.cpp.o")]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"cli build failed\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     if verbose:
-        print(f"built {LIB} ({LIB.stat().st_size // 1024} KiB) from {len(srcs)} sources for {ARCH}")
+        print(f"built {LIB} ({LIB.stat().st_size // 1024} KiB) from {len(srcs)} sources for {ARCH}, and {exe.name}")
     return LIB
 
 

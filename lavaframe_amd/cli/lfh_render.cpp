@@ -1,0 +1,115 @@
+// Headless render CLI with the reference's command-line surface for batch
+// rendering (LavaFrame/Main.cpp:768-958): -s/--scene, -ms/--maxsamples,
+// -ou/--outputpath, -ep/-exportname, -et/--exporttype, -db/--debug,
+// -u/-nm/-w/-n (window / UI flags, accepted and ignored: always headless).
+// It loads the scene with this build's loader, renders with HipRendererT
+// through the C ABI until maxSamples passes are complete, and writes the
+// linear HDR image.  Export formats: pfm (linear RGB float) and ppm (RGB8,
+// clamped); the reference's png/jpg/tga/bmp/exr writers are display-side
+// and outside this build.
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../csrc/host_scene.h"
+#include "lfh.h"
+#include "lfh_renderer.hpp"
+
+int main(int argc, char** argv) {
+    std::string scene_file, out_path = "./", name = "", type = "pfm";
+    int max_samples = -1;
+    bool debug = false;
+    for (int i = 1; i < argc; ++i) {
+        std::string a = argv[i];
+        auto next = [&](const char* what) -> std::string {
+            if (i + 1 >= argc) {
+                std::fprintf(stderr, "missing value for %s\n", what);
+                std::exit(2);
+            }
+            return argv[++i];
+        };
+        if (a == "-s" || a == "--scene") scene_file = next("-s");
+        else if (a == "-ms" || a == "--maxsamples") max_samples = std::atoi(next("-ms").c_str());
+        else if (a == "-ou" || a == "--outputpath") out_path = next("-ou");
+        else if (a == "-ep" || a == "-exportname") name = next("-ep");
+        else if (a == "-et" || a == "--exporttype") type = next("-et");
+        else if (a == "-db" || a == "--debug") debug = true;
+        else if (a == "-u" || a == "--noui" || a == "-nm" || a == "--nomove" || a == "-w" || a == "--nowindow" ||
+                 a == "-n" || a == "--neutral") {
+        } else if (a == "-h" || a == "--help") {
+            std::printf("lfh_render -s scene.lfs -ms samples [-ou dir] [-ep name] [-et pfm|ppm] [-db]\n");
+            return 0;
+        } else {
+            std::fprintf(stderr, "unknown argument %s\n", a.c_str());
+            return 2;
+        }
+    }
+    if (scene_file.empty()) {
+        std::fprintf(stderr, "no scene given (-s)\n");
+        return 2;
+    }
+    if (max_samples <= 0) max_samples = 64;
+    lfh::Scene scene;
+    std::string err;
+    if (!lfh::LoadSceneFromFile(scene_file, &scene, scene.renderOptions, &err)) {
+        std::fprintf(stderr, "could not load %s: %s\n", scene_file.c_str(), err.c_str());
+        return 1;
+    }
+    scene.camera->isMoving = false;
+    lfh::HipRendererT<lfh::Scene> renderer(&scene);
+    try {
+        renderer.Init();
+        auto t0 = std::chrono::steady_clock::now();
+        // Main loop order (Main.cpp:747-753): Update, then Render; export when
+        // the sample counter reaches maxSamples + 1, i.e. after maxSamples passes.
+        for (;;) {
+            renderer.Update(0.f);
+            if (renderer.GetSampleCount() == max_samples + 1) break;
+            renderer.Render();
+        }
+        lfh_synchronize(renderer.context());
+        double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (debug) std::printf("%d samples in %.3f s\n", max_samples, s);
+        float* hdr = nullptr;
+        int w = 0, h = 0;
+        // export after maxSamples complete passes: accum / maxSamples
+        renderer.GetOutputBufferHDR(&hdr, w, h);
+        float scale = (float)(max_samples + 1) / (float)max_samples;
+        if (name.empty()) {
+            size_t p = scene_file.find_last_of("/\\");
+            name = scene_file.substr(p == std::string::npos ? 0 : p + 1);
+            size_t d = name.find_last_of('.');
+            if (d != std::string::npos) name = name.substr(0, d);
+        }
+        std::string path = out_path + (out_path.empty() || out_path.back() == '/' ? "" : "/") + name + "." + type;
+        FILE* f = std::fopen(path.c_str(), "wb");
+        if (!f) {
+            std::fprintf(stderr, "cannot write %s\n", path.c_str());
+            delete[] hdr;
+            return 1;
+        }
+        if (type == "ppm") {
+            std::fprintf(f, "P6\n%d %d\n255\n", w, h);
+            for (int y = h - 1; y >= 0; --y)
+                for (int x = 0; x < w; ++x)
+                    for (int c = 0; c < 3; ++c) {
+                        float v = hdr[((size_t)y * w + x) * 3 + c] * scale;
+                        v = v < 0.f ? 0.f : (v > 1.f ? 1.f : v);
+                        std::fputc((int)(v * 255.f + 0.5f), f);
+                    }
+        } else {
+            // PFM rows are bottom-to-top, like the GL readback
+            std::fprintf(f, "PF\n%d %d\n-1.0\n", w, h);
+            for (size_t i = 0; i < (size_t)w * h * 3; ++i) hdr[i] *= scale;
+            std::fwrite(hdr, sizeof(float), (size_t)w * h * 3, f);
+        }
+        std::fclose(f);
+        delete[] hdr;
+        std::printf("%s\n", path.c_str());
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "%s\n", e.what());
+        return 1;
+    }
+    return 0;
+}

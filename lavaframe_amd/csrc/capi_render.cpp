@@ -31,7 +31,7 @@ static int rhip(lfh_ctx* c, hipError_t e, const char* where) {
 
 namespace lfh {
 
-static PathBufs view(const PassBufs& b) {
+PathBufs view(const PassBufs& b) {
     PathBufs p;
     p.rad = b.rad.as<float4>();
     p.thr = b.thr.as<float4>();
@@ -39,6 +39,7 @@ static PathBufs view(const PassBufs& b) {
     p.rng = b.rng.as<uint4>();
     p.misc = b.misc.as<float4>();
     p.nee = b.nee.as<float4>();
+    p.job = b.job.as<int>();
     return p;
 }
 

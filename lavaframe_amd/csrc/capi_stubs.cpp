@@ -10,14 +10,6 @@ static int unsupported(const char* fn) {
 }
 
 extern "C" {
-int lfh_server_create(int64_t, int32_t, lfh_server**) { return unsupported("lfh_server_create"); }
-void lfh_server_destroy(lfh_server*) {}
-int lfh_server_submit(lfh_server*, const lfh_job_desc*, int64_t*) { return unsupported("lfh_server_submit"); }
-int lfh_server_step(lfh_server*, int32_t*) { return unsupported("lfh_server_step"); }
-int lfh_server_status(lfh_server*, int64_t, lfh_job_status*) { return unsupported("lfh_server_status"); }
-int lfh_server_result(lfh_server*, int64_t, float*, int32_t, int32_t) { return unsupported("lfh_server_result"); }
-int lfh_server_release(lfh_server*, int64_t) { return unsupported("lfh_server_release"); }
-double lfh_server_clock(const lfh_server*) { return 0.0; }
 int lfh_guide_create(const float*, const float*, uint64_t, float, lfh_guide**) { return unsupported("lfh_guide_create"); }
 void lfh_guide_destroy(lfh_guide*) {}
 int lfh_guide_get_params(lfh_guide*, float*, int64_t*) { return unsupported("lfh_guide_get_params"); }

@@ -68,6 +68,11 @@ std::string repack_instances(PackedScene* ps, const float* transforms, const flo
 
 struct DeviceScene {
     DevBuf nodes, tris, insts, xforms, nmats, vidx, normals, materials, lights, hdr_cols, hdr_marg, hdr_cond;
+    void release() {
+        DevBuf* b[12] = {&nodes, &tris, &insts, &xforms, &nmats, &vidx, &normals, &materials, &lights,
+                         &hdr_cols, &hdr_marg, &hdr_cond};
+        for (DevBuf* x : b) x->release();
+    }
     size_t bytes() const {
         return nodes.bytes + tris.bytes + insts.bytes + xforms.bytes + nmats.bytes + vidx.bytes + normals.bytes +
                materials.bytes + lights.bytes + hdr_cols.bytes + hdr_marg.bytes + hdr_cond.bytes;
@@ -86,6 +91,7 @@ struct PassBufs {
     DevBuf rad, thr, absorb, rng, misc, nee;  // per path (nee: 3 float4)
     DevBuf rq0, rq1, hq, sq;                  // ray queues, hit queue, shadow queue
     DevBuf counters, work;
+    DevBuf job;                               // int per path (multi-job passes)
     int cap = 0;
     hipError_t alloc(int n) {
         hipError_t e;
@@ -96,12 +102,18 @@ struct PassBufs {
             if ((e = b[i]->alloc(f4 * m[i] * (size_t)n)) != hipSuccess) return e;
         if ((e = counters.alloc(64)) != hipSuccess) return e;
         if ((e = work.alloc(64)) != hipSuccess) return e;
+        if ((e = job.alloc(4 * (size_t)n)) != hipSuccess) return e;
         cap = n;
         return hipSuccess;
     }
+    void release() {
+        DevBuf* b[13] = {&rad, &thr, &absorb, &rng, &misc, &nee, &rq0, &rq1, &hq, &sq, &counters, &work, &job};
+        for (DevBuf* x : b) x->release();
+        cap = 0;
+    }
     size_t bytes() const {
         return rad.bytes + thr.bytes + absorb.bytes + rng.bytes + misc.bytes + nee.bytes + rq0.bytes + rq1.bytes +
-               hq.bytes + sq.bytes + counters.bytes + work.bytes;
+               hq.bytes + sq.bytes + counters.bytes + work.bytes + job.bytes;
     }
 };
 

@@ -70,14 +70,17 @@ LFH_HD float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 LFH_HD V3 cross(V3 a, V3 b) {
     return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
+#if defined(__HIP__)
 // normalize(v) = v / sqrt(dot(v, v))
 LFH_DEV V3 normalize(V3 a) {
     float l = sqrtf(dot(a, a));
     return v3(a.x / l, a.y / l, a.z / l);
 }
 LFH_DEV float length(V3 a) { return sqrtf(dot(a, a)); }
+#endif
 LFH_HD V3 xyz(float4 f) { return v3(f.x, f.y, f.z); }
 
+#if defined(__HIP__)
 LFH_DEV float as_f(uint32_t u) { return __uint_as_float(u); }
 LFH_DEV uint32_t as_u(float f) { return __float_as_uint(f); }
 
@@ -124,5 +127,6 @@ LFH_DEV V3 xform_inv(const float4& r0, const float4& r1, const float4& r2, V3 p,
 }
 
 LFH_DEV V3 rcp3(V3 d) { return v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z); }
+#endif  // __HIP__
 
 }  // namespace lfhd

@@ -420,12 +420,7 @@ LFH_DEV void EnvSample(const ShadeScene& sc, uint4& rng, V3& color, V3& dir, flo
 }
 
 // ---- camera ray for one tile pixel (renderer.glsl:25-62) ----------------------------
-struct CamParams {
-    float pos[3], right[3], up[3], fwd[3];
-    float fov, focal_dist, aperture;
-    int width, height, tile_w, tile_h, tileX, tileY;
-    int frame;
-};
+
 
 LFH_DEV float mapf(float value, float low1, float high1, float low2, float high2) {
     return low2 + ((value - low1) * (high2 - low2)) / (high1 - low1);

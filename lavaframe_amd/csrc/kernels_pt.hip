@@ -19,6 +19,15 @@
 
 namespace lfhd {
 
+__device__ __forceinline__ void guide_record(const GuideHook& g, int p, int depth, V3 x, V3 n, V3 wo,
+                                             V3 wi, float pdf, V3 thr, bool alive) {
+    float4* r = g.rec + ((size_t)p * g.max_depth + depth) * 4;
+    r[0] = make_float4(x.x, x.y, x.z, alive ? 1.0f : 0.0f);
+    r[1] = make_float4(n.x, n.y, n.z, pdf);
+    r[2] = make_float4(wo.x, wo.y, wo.z, 0.0f);
+    r[3] = make_float4(wi.x, wi.y, wi.z, (thr.x + thr.y + thr.z) * (1.0f / 3.0f));
+}
+
 // ---- queue helpers ---------------------------------------------------------------
 // Wave-aggregated append: returns this lane's slot (valid only if `want`).
 LFH_DEV uint32_t wave_append(uint32_t* counter, bool want) {
@@ -48,16 +57,27 @@ __global__ void k_pt_advance(uint32_t* consumed, uint32_t* shadow_count, unsigne
     if (threadIdx.x < 8) work[threadIdx.x] = 0ull;
 }
 
-// generate: one thread per tile pixel
-__global__ __launch_bounds__(256) void k_generate(CamParams cam, PathBufs pb, float4* __restrict__ rayq,
+// generate: one thread per tile pixel (MULTI: per pixel of every job)
+template <bool MULTI>
+__global__ __launch_bounds__(256) void k_generate(CamParams cam1, const JobDev* __restrict__ jobs, int njobs,
+                                                  int total, PathBufs pb, float4* __restrict__ rayq,
                                                   uint32_t* __restrict__ count) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    int lp = p, j = 0;
+    CamParams cam = cam1;
+    bool inrange = p < total;
+    if (MULTI && inrange) {
+        while (j + 1 < njobs && jobs[j + 1].path_offset <= p) ++j;
+        cam = jobs[j].cam;
+        lp = p - jobs[j].path_offset;
+        pb.job[p] = j;
+    }
     const int P = cam.tile_w * cam.tile_h;
     bool valid = false;
     int lx = 0, ly = 0;
-    if (p < P) {
-        lx = p % cam.tile_w;
-        ly = p / cam.tile_w;
+    if (inrange && lp < P) {
+        lx = lp % cam.tile_w;
+        ly = lp / cam.tile_w;
         valid = (cam.tileX * cam.tile_w + lx < cam.width) && (cam.tileY * cam.tile_h + ly < cam.height);
     }
     V3 o = v3(0.f, 0.f, 0.f), d = v3(0.f, 0.f, 1.f);
@@ -70,7 +90,7 @@ __global__ __launch_bounds__(256) void k_generate(CamParams cam, PathBufs pb, fl
         pb.rng[p] = seed;
         pb.misc[p] = make_float4(0.f, __int_as_float(0), 0.f, 0.f);
         pb.nee[p * 3 + 2] = make_float4(0.f, 0.f, 0.f, 0.f);
-    } else if (p < P) {
+    } else if (inrange) {
         pb.rad[p] = make_float4(0.f, 0.f, 0.f, 0.f);
         pb.nee[p * 3 + 2] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
@@ -149,8 +169,9 @@ LFH_DEV bool light_any(const ShadeScene& sc, V3 o, V3 d, float maxDist) {
     return false;
 }
 
-template <bool SHADOW>
-__global__ __launch_bounds__(kTraceBlock) void k_trace_queue(ShadeScene sc, const float4* __restrict__ q,
+template <bool SHADOW, bool MULTI>
+__global__ __launch_bounds__(kTraceBlock) void k_trace_queue(ShadeScene sc, const JobDev* __restrict__ jobs,
+                                                             const float4* __restrict__ q,
                                                              const uint32_t* __restrict__ count,
                                                              float4* __restrict__ hitq, PathBufs pb,
                                                              uint32_t* __restrict__ spill,
@@ -170,6 +191,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_queue(ShadeScene sc, cons
     int64_t rid = 0;
     int light = -1;
     float lpdf = 0.f;
+    TraceScene ts = sc.tr;
     for (;;) {
         const bool need = !active;
         const uint64_t mask = __ballot(need);
@@ -183,10 +205,14 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_queue(ShadeScene sc, cons
                 if (rid < n) {
                     float4 a = q[rid * 2], b = q[rid * 2 + 1];
                     V3 o = v3(a.x, a.y, a.z), d = v3(b.x, b.y, b.z);
+                    const uint32_t tag = __float_as_uint(a.w);
+                    const int pth = SHADOW ? (int)(tag >> 1) : (int)tag;
+                    const ShadeScene& S = MULTI ? jobs[pb.job[pth]].sc : sc;
+                    ts = S.tr;
                     if (SHADOW) {
                         float maxd = b.w;
-                        trace_init(s, st, o, d, 0.0f, maxd, sc.tr.root);
-                        if (light_any(sc, o, d, maxd)) {
+                        trace_init(s, st, o, d, 0.0f, maxd, ts.root);
+                        if (light_any(S, o, d, maxd)) {
                             s.occluded = true;
                             s.cur = kRefSentinel;  // finishes at once
                         }
@@ -194,8 +220,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_queue(ShadeScene sc, cons
                         float t = kInfinity;
                         light = -1;
                         lpdf = 0.f;
-                        light_closest(sc, o, d, t, light, lpdf);
-                        trace_init(s, st, o, d, 0.0f, t, sc.tr.root);
+                        light_closest(S, o, d, t, light, lpdf);
+                        trace_init(s, st, o, d, 0.0f, t, ts.root);
                     }
                     active = true;
                 }
@@ -203,7 +229,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_queue(ShadeScene sc, cons
         }
         if (__ballot(active) == 0) break;
         if (active) {
-            bool done = trace_run<SHADOW, false>(sc.tr, s, st, cull, 48, nullptr);
+            bool done = trace_run<SHADOW, false>(ts, s, st, cull, 48, nullptr);
             if (done) {
                 if (st.overflow) atomicOr(err, 1);
                 if (SHADOW) {
@@ -250,7 +276,9 @@ LFH_DEV void onb(V3 N, V3& T, V3& B) {
     B = cross(N, T);
 }
 
-__global__ __launch_bounds__(256) void k_shade(ShadeScene sc, int depth, const float4* __restrict__ rayq,
+template <bool MULTI>
+__global__ __launch_bounds__(256) void k_shade(ShadeScene sc1, const JobDev* __restrict__ jobs, int depth,
+                                               const float4* __restrict__ rayq,
                                                const float4* __restrict__ hitq, const uint32_t* __restrict__ count,
                                                float4* __restrict__ nextq, uint32_t* __restrict__ next_count,
                                                float4* __restrict__ shadowq, uint32_t* __restrict__ shadow_count,
@@ -267,6 +295,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeScene sc, int depth, const f
     if (live) {
         float4 r0 = rayq[(size_t)i * 2], r1 = rayq[(size_t)i * 2 + 1];
         p = __float_as_int(r0.w);
+        const ShadeScene& sc = MULTI ? jobs[pb.job[p]].sc : sc1;
         V3 ro = ld3(r0), rd = ld3(r1);
         float4 h0 = hitq[(size_t)i * 2], h1 = hitq[(size_t)i * 2 + 1];
         V3 rad = ld3(pb.rad[p]);
@@ -428,17 +457,30 @@ __global__ __launch_bounds__(256) void k_shade(ShadeScene sc, int depth, const f
 }
 
 // ---- accumulate: pending NEE + Kahan sum into the frame -----------------------------
-__global__ __launch_bounds__(256) void k_accumulate(PathBufs pb, int P, int tile_w, int tileX, int tileY, int tile_h,
-                                                    int width, int height, float4* __restrict__ accum,
-                                                    float4* __restrict__ comp, int kahan) {
+template <bool MULTI>
+__global__ __launch_bounds__(256) void k_accumulate(PathBufs pb, int total, CamParams cam1, float4* accum1,
+                                                    float4* comp1, int kahan1, const JobDev* __restrict__ jobs) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= P) return;
-    const int lx = p % tile_w, ly = p / tile_w;
-    const int gx = tileX * tile_w + lx, gy = tileY * tile_h + ly;
-    if (gx >= width || gy >= height) return;
+    if (p >= total) return;
+    CamParams cam = cam1;
+    float4* accum = accum1;
+    float4* comp = comp1;
+    int kahan = kahan1, lp = p;
+    if (MULTI) {
+        const JobDev& J = jobs[pb.job[p]];
+        cam = J.cam;
+        accum = J.accum;
+        comp = J.comp;
+        kahan = J.kahan;
+        lp = p - J.path_offset;
+    }
+    if (lp >= cam.tile_w * cam.tile_h) return;
+    const int lx = lp % cam.tile_w, ly = lp / cam.tile_w;
+    const int gx = cam.tileX * cam.tile_w + lx, gy = cam.tileY * cam.tile_h + ly;
+    if (gx >= cam.width || gy >= cam.height) return;
     V3 rad = ld3(pb.rad[p]);
     combine_nee(pb, p, rad);
-    const size_t pix = (size_t)gy * width + gx;
+    const size_t pix = (size_t)gy * cam.width + gx;
     float4 s = accum[pix];
     if (kahan) {
         float4 c = comp[pix];
@@ -582,7 +624,16 @@ hipError_t pt_reset(uint32_t* counters, unsigned long long* work, hipStream_t s)
 
 hipError_t pt_generate(const CamParams& cam, const PathBufs& pb, float4* rayq, uint32_t* count, hipStream_t s) {
     int P = cam.tile_w * cam.tile_h;
-    hipLaunchKernelGGL(k_generate, dim3(nblk(P)), dim3(256), 0, s, cam, pb, rayq, count);
+    hipLaunchKernelGGL(k_generate<false>, dim3(nblk(P)), dim3(256), 0, s, cam, (const JobDev*)nullptr, 1, P, pb, rayq,
+                       count);
+    return hipGetLastError();
+}
+
+hipError_t pt_generate_multi(const JobDev* jobs, int njobs, int total, const PathBufs& pb, float4* rayq,
+                             uint32_t* count, hipStream_t s) {
+    CamParams dummy{};
+    hipLaunchKernelGGL(k_generate<true>, dim3(nblk(total)), dim3(256), 0, s, dummy, jobs, njobs, total, pb, rayq,
+                       count);
     return hipGetLastError();
 }
 
@@ -596,21 +647,45 @@ hipError_t pt_bounce(const ShadeScene& sc, int depth, int maxq, float4* rayq, ui
                      uint32_t* next_count, float4* hitq, float4* shadowq, uint32_t* shadow_count, const PathBufs& pb,
                      uint32_t* spill, unsigned long long* work, int* err, float cull, int grid_blocks,
                      const GuideHook& gh, hipStream_t s) {
-    hipLaunchKernelGGL((k_trace_queue<false>), dim3(grid_blocks), dim3(kTraceBlock), 0, s, sc, rayq, count, hitq, pb,
-                       spill, work, cull, err);
-    hipLaunchKernelGGL(k_shade, dim3(nblk(maxq)), dim3(256), 0, s, sc, depth, rayq, hitq, count, nextq, next_count,
-                       shadowq, shadow_count, pb, gh);
-    hipLaunchKernelGGL((k_trace_queue<true>), dim3(grid_blocks), dim3(kTraceBlock), 0, s, sc, shadowq, shadow_count,
-                       hitq, pb, spill, work + 1, cull, err);
+    const JobDev* nj = nullptr;
+    hipLaunchKernelGGL((k_trace_queue<false, false>), dim3(grid_blocks), dim3(kTraceBlock), 0, s, sc, nj, rayq, count,
+                       hitq, pb, spill, work, cull, err);
+    hipLaunchKernelGGL(k_shade<false>, dim3(nblk(maxq)), dim3(256), 0, s, sc, nj, depth, rayq, hitq, count, nextq,
+                       next_count, shadowq, shadow_count, pb, gh);
+    hipLaunchKernelGGL((k_trace_queue<true, false>), dim3(grid_blocks), dim3(kTraceBlock), 0, s, sc, nj, shadowq,
+                       shadow_count, hitq, pb, spill, work + 1, cull, err);
     hipLaunchKernelGGL(k_pt_advance, dim3(1), dim3(64), 0, s, count, shadow_count, work);
+    return hipGetLastError();
+}
+
+hipError_t pt_bounce_multi(const JobDev* jobs, int depth, int maxq, float4* rayq, uint32_t* count, float4* nextq,
+                           uint32_t* next_count, float4* hitq, float4* shadowq, uint32_t* shadow_count,
+                           const PathBufs& pb, uint32_t* spill, unsigned long long* work, int* err, float cull,
+                           int grid_blocks, hipStream_t s) {
+    ShadeScene dummy{};
+    GuideHook gh{0, 0, nullptr};
+    hipLaunchKernelGGL((k_trace_queue<false, true>), dim3(grid_blocks), dim3(kTraceBlock), 0, s, dummy, jobs, rayq,
+                       count, hitq, pb, spill, work, cull, err);
+    hipLaunchKernelGGL(k_shade<true>, dim3(nblk(maxq)), dim3(256), 0, s, dummy, jobs, depth, rayq, hitq, count, nextq,
+                       next_count, shadowq, shadow_count, pb, gh);
+    hipLaunchKernelGGL((k_trace_queue<true, true>), dim3(grid_blocks), dim3(kTraceBlock), 0, s, dummy, jobs, shadowq,
+                       shadow_count, hitq, pb, spill, work + 1, cull, err);
+    hipLaunchKernelGGL(k_pt_advance, dim3(1), dim3(64), 0, s, count, shadow_count, work);
+    return hipGetLastError();
+}
+
+hipError_t pt_accumulate_multi(const JobDev* jobs, int total, const PathBufs& pb, hipStream_t s) {
+    CamParams dummy{};
+    hipLaunchKernelGGL(k_accumulate<true>, dim3(nblk(total)), dim3(256), 0, s, pb, total, dummy, (float4*)nullptr,
+                       (float4*)nullptr, 0, jobs);
     return hipGetLastError();
 }
 
 hipError_t pt_accumulate(const PathBufs& pb, const CamParams& cam, float4* accum, float4* comp, int kahan,
                          hipStream_t s) {
     int P = cam.tile_w * cam.tile_h;
-    hipLaunchKernelGGL(k_accumulate, dim3(nblk(P)), dim3(256), 0, s, pb, P, cam.tile_w, cam.tileX, cam.tileY,
-                       cam.tile_h, cam.width, cam.height, accum, comp, kahan);
+    hipLaunchKernelGGL(k_accumulate<false>, dim3(nblk(P)), dim3(256), 0, s, pb, P, cam, accum, comp, kahan,
+                       (const JobDev*)nullptr);
     return hipGetLastError();
 }
 
@@ -637,10 +712,10 @@ int pt_trace_grid_blocks() {
         hipGetDevice(&dev);
         hipDeviceProp_t prop;
         hipGetDeviceProperties(&prop, dev);
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, (const void*)k_trace_queue<false>, kTraceBlock, 0) !=
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, (const void*)k_trace_queue<false, true>, kTraceBlock, 0) !=
                 hipSuccess || a <= 0)
             a = 4;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, (const void*)k_trace_queue<true>, kTraceBlock, 0) !=
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, (const void*)k_trace_queue<false, false>, kTraceBlock, 0) !=
                 hipSuccess || b <= 0)
             b = 4;
         blocks = (a > b ? a : b) * prop.multiProcessorCount;

@@ -3,7 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#include "dev_shade.h"
+#include "pt_types.h"
 
 // Path state, indexed by path id (tile pixel).
 struct PathBufs {
@@ -13,6 +13,7 @@ struct PathBufs {
     uint4* rng;      // pcg4d state
     float4* misc;    // (previous BSDF pdf, last material id, 0, 0)
     float4* nee;     // 3 per path: env term, light term (w = 1 while unoccluded), (throughput, pending)
+    int* job;        // job id per path (multi-job passes only)
 };
 
 // Training-sample hook of the learned guide (disabled when enabled == 0):
@@ -24,14 +25,6 @@ struct GuideHook {
     float4* rec;     // 4 float4 per (path, depth)
 };
 
-__device__ __forceinline__ void guide_record(const GuideHook& g, int p, int depth, lfhd::V3 x, lfhd::V3 n, lfhd::V3 wo,
-                                             lfhd::V3 wi, float pdf, lfhd::V3 thr, bool alive) {
-    float4* r = g.rec + ((size_t)p * g.max_depth + depth) * 4;
-    r[0] = make_float4(x.x, x.y, x.z, alive ? 1.0f : 0.0f);
-    r[1] = make_float4(n.x, n.y, n.z, pdf);
-    r[2] = make_float4(wo.x, wo.y, wo.z, 0.0f);
-    r[3] = make_float4(wi.x, wi.y, wi.z, (thr.x + thr.y + thr.z) * (1.0f / 3.0f));
-}
 
 hipError_t pt_reset(uint32_t* counters, unsigned long long* work, hipStream_t s);
 hipError_t pt_generate(const lfhd::CamParams& cam, const PathBufs& pb, float4* rayq, uint32_t* count, hipStream_t s);
@@ -41,6 +34,15 @@ hipError_t pt_bounce(const lfhd::ShadeScene& sc, int depth, int maxq, float4* ra
                      uint32_t* next_count, float4* hitq, float4* shadowq, uint32_t* shadow_count, const PathBufs& pb,
                      uint32_t* spill, unsigned long long* work, int* err, float cull, int grid_blocks,
                      const GuideHook& gh, hipStream_t s);
+// Multi-job pass (serving): every job renders its full frame once; rays of all
+// jobs share the queues.  jobs: device table of njobs entries.
+hipError_t pt_generate_multi(const lfhd::JobDev* jobs, int njobs, int total, const PathBufs& pb, float4* rayq,
+                             uint32_t* count, hipStream_t s);
+hipError_t pt_bounce_multi(const lfhd::JobDev* jobs, int depth, int maxq, float4* rayq, uint32_t* count,
+                           float4* nextq, uint32_t* next_count, float4* hitq, float4* shadowq, uint32_t* shadow_count,
+                           const PathBufs& pb, uint32_t* spill, unsigned long long* work, int* err, float cull,
+                           int grid_blocks, hipStream_t s);
+hipError_t pt_accumulate_multi(const lfhd::JobDev* jobs, int total, const PathBufs& pb, hipStream_t s);
 hipError_t pt_accumulate(const PathBufs& pb, const lfhd::CamParams& cam, float4* accum, float4* comp, int kahan,
                          hipStream_t s);
 hipError_t pt_collect(const PathBufs& pb, int n, float* out, hipStream_t s);

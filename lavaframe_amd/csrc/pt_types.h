@@ -35,4 +35,24 @@ struct ShadeScene {
     float bg[3];
 };
 
+// Camera uniforms + tile / frame of one pass (renderer.glsl inputs).
+struct CamParams {
+    float pos[3], right[3], up[3], fwd[3];
+    float fov, focal_dist, aperture;
+    int width, height, tile_w, tile_h, tileX, tileY;
+    int frame;
+};
+
+// One job of a multi-job (serving) pass: its scene, camera and frame, its
+// accumulation buffers, and the first global path id of its pixels.
+struct JobDev {
+    ShadeScene sc;
+    CamParams cam;
+    float4* accum;
+    float4* comp;
+    int path_offset;
+    int kahan;
+    int pad[2];
+};
+
 }  // namespace lfhd

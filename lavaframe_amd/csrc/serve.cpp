@@ -1,0 +1,407 @@
+// Render service: continuous batching of path-tracing jobs on one GPU.
+//
+// No reference counterpart (the reference renders one interactive scene).
+// Jobs arrive with their scene arrays, options, camera and a target sample
+// count.  Each lfh_server_step():
+//   1. admits queued jobs (priority, then arrival order; a job that does not
+//      fit is skipped so smaller ones can backfill) while the device budget
+//      and the job slots allow;
+//   2. runs ONE wavefront pass for all running jobs together: the rays of
+//      every job go through the same extend / shade / shadow queues and the
+//      same persistent kernels (the per-path job id selects scene, camera and
+//      accumulation buffers);
+//   3. retires jobs that reached their target.
+// Device scenes are shared by content hash and stay resident after their
+// jobs finish, until the budget needs the space (least recently used first).
+// A job's result is bit-identical whether it runs alone or in a mix: its
+// paths depend only on its own scene, camera, pixel and frame numbers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "capi_common.h"
+#include "ctx.h"
+#include "kernels.h"
+#include "pt_kernels.h"
+#include "render.h"
+
+namespace lfh {
+
+namespace {
+
+// Device bytes of one pass's path state + queues per path (PassBufs).
+constexpr int64_t kPathBytes = 16 * (1 + 1 + 1 + 1 + 1 + 3 + 2 + 2 + 2 + 4) + 4;
+constexpr int64_t kAccumBytes = 32;  // sum + compensation, float4 each
+
+uint64_t fnv(uint64_t h, const void* p, size_t n) {
+    const unsigned char* c = static_cast<const unsigned char*>(p);
+    for (size_t i = 0; i < n; ++i) {
+        h ^= c[i];
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+uint64_t hash_desc(const lfh_scene_desc& d) {
+    uint64_t h = 1469598103934665603ull;
+    h = fnv(h, d.nodes, (size_t)d.num_nodes * 36);
+    h = fnv(h, &d.top_level_index, 4);
+    h = fnv(h, d.vert_indices, (size_t)d.num_tris * 12);
+    h = fnv(h, d.vertices_uvx, (size_t)d.num_vertices * 16);
+    if (d.normals_uvy) h = fnv(h, d.normals_uvy, (size_t)d.num_vertices * 16);
+    h = fnv(h, d.materials, (size_t)d.num_materials * 112);
+    h = fnv(h, d.transforms, (size_t)d.num_instances * 64);
+    if (d.lights) h = fnv(h, d.lights, (size_t)d.num_lights * 60);
+    if (d.hdr_width > 0) {
+        size_t n = (size_t)d.hdr_width * d.hdr_height;
+        h = fnv(h, d.hdr_cols, n * 12);
+        h = fnv(h, d.hdr_marginal, (size_t)d.hdr_height * 8);
+        h = fnv(h, d.hdr_conditional, n * 8);
+    }
+    return h;
+}
+
+int64_t packed_bytes(const PackedScene& ps) {
+    return (int64_t)(ps.nodes.size() + ps.tris.size() + ps.insts.size() + ps.xforms.size() + ps.nmats.size() +
+                     ps.normals.size() + ps.materials.size() + ps.lights.size() + ps.hdr_cols.size() +
+                     ps.hdr_marg.size() + ps.hdr_cond.size()) * 4 +
+           (int64_t)ps.vidx.size() * 4;
+}
+
+struct SceneEntry {
+    uint64_t hash = 0;
+    PackedScene ps;
+    DeviceScene ds;
+    bool resident = false;
+    int refs = 0;
+    int64_t bytes = 0;
+    double last_use = 0;
+};
+
+struct Job {
+    int64_t id = 0;
+    int state = 0;  // 0 queued, 1 running, 2 done, -1 rejected
+    int spp_done = 0, target = 0, priority = 0;
+    int64_t tag = 0;
+    double t_submit = 0, t_admit = 0, t_done = 0;
+    lfh_options opt{};
+    lfh_camera cam{};
+    std::shared_ptr<SceneEntry> scene;
+    DevBuf accum, comp;
+    int frame = 2;  // first pass uses frame 2, like TiledRenderer after Init + Update
+    int64_t npix = 0;
+    int64_t charged = 0;  // accum + path share held while running
+};
+
+}  // namespace
+}  // namespace lfh
+
+struct lfh_server {
+    int64_t budget = 0;
+    int max_jobs = 0;
+    hipStream_t stream = nullptr;
+    std::chrono::steady_clock::time_point t0;
+    int64_t next_id = 1;
+    std::map<int64_t, std::unique_ptr<lfh::Job>> jobs;
+    std::map<uint64_t, std::shared_ptr<lfh::SceneEntry>> scenes;
+    lfh::PassBufs pass;
+    lfh::DevBuf table, spill, errflag;
+    int grid = 0;
+    int64_t used = 0;  // resident scenes + accum of admitted jobs + path shares of running jobs
+    std::string err;
+    double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
+};
+
+using lfh::DevBuf;
+
+static int sfail(lfh_server* s, int code, const std::string& m) {
+    if (s) s->err = m;
+    lfh_internal::set_thread_error(m);
+    return code;
+}
+static int shipfail(lfh_server* s, hipError_t e, const char* where) {
+    return sfail(s, e == hipErrorOutOfMemory ? LFH_E_OOM : LFH_E_HIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+#define SCHK(s, expr, where)                                  \
+    do {                                                      \
+        hipError_t _e = (expr);                               \
+        if (_e != hipSuccess) return shipfail(s, _e, where);  \
+    } while (0)
+
+// Evict unreferenced resident scenes (oldest first) until `need` more bytes fit.
+static void evict_for(lfh_server* s, int64_t need) {
+    while (s->used + need > s->budget) {
+        std::shared_ptr<lfh::SceneEntry> victim;
+        for (auto& kv : s->scenes) {
+            auto& e = kv.second;
+            if (e->resident && e->refs == 0 && (!victim || e->last_use < victim->last_use)) victim = e;
+        }
+        if (!victim) return;
+        victim->ds.release();
+        victim->resident = false;
+        s->used -= victim->bytes;
+    }
+}
+
+extern "C" {
+
+int lfh_server_create(int64_t budget_bytes, int32_t max_jobs, lfh_server** out) {
+    if (!out || max_jobs <= 0 || max_jobs > 64) return sfail(nullptr, LFH_E_INVALID, "lfh_server_create: bad arguments");
+    *out = nullptr;
+    lfh_server* s = new (std::nothrow) lfh_server;
+    if (!s) return sfail(nullptr, LFH_E_OOM, "lfh_server_create: out of host memory");
+    s->t0 = std::chrono::steady_clock::now();
+    s->max_jobs = max_jobs;
+    if (budget_bytes <= 0) {
+        size_t f = 0, t = 0;
+        if (hipMemGetInfo(&f, &t) != hipSuccess) {
+            delete s;
+            return sfail(nullptr, LFH_E_HIP, "lfh_server_create: hipMemGetInfo failed");
+        }
+        budget_bytes = (int64_t)(0.9 * (double)f);
+    }
+    s->budget = budget_bytes;
+    hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) {
+        s->grid = pt_trace_grid_blocks();
+        e = s->spill.alloc((size_t)s->grid * lfh_trace_threads_per_block() * lfh_trace_spill_stride() * 4);
+    }
+    if (e == hipSuccess) e = s->errflag.alloc(64);
+    if (e == hipSuccess) e = hipMemsetAsync(s->errflag.p, 0, 64, s->stream);
+    if (e == hipSuccess) e = s->table.alloc(sizeof(lfhd::JobDev) * 64);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) {
+        int r = shipfail(nullptr, e, "lfh_server_create");
+        lfh_server_destroy(s);
+        return r;
+    }
+    *out = s;
+    return LFH_OK;
+}
+
+void lfh_server_destroy(lfh_server* s) {
+    if (!s) return;
+    if (s->stream) hipStreamSynchronize(s->stream);
+    s->jobs.clear();
+    s->scenes.clear();
+    s->pass.release();
+    s->table.release();
+    s->spill.release();
+    s->errflag.release();
+    if (s->stream) hipStreamDestroy(s->stream);
+    delete s;
+}
+
+double lfh_server_clock(const lfh_server* s) { return s ? s->now() : 0.0; }
+
+int lfh_server_submit(lfh_server* s, const lfh_job_desc* jd, int64_t* job_id) {
+    if (!s || !jd || !jd->scene) return sfail(s, LFH_E_INVALID, "lfh_server_submit: NULL argument");
+    const lfh_options& o = jd->options;
+    if (o.width <= 0 || o.height <= 0 || o.max_depth < 0 || jd->target_spp <= 0)
+        return sfail(s, LFH_E_INVALID, "lfh_server_submit: bad resolution, depth or target spp");
+    if (o.guiding) return sfail(s, LFH_E_UNSUPPORTED, "lfh_server_submit: guided jobs are not served");
+    auto job = std::make_unique<lfh::Job>();
+    job->id = s->next_id++;
+    job->opt = o;
+    job->cam = jd->camera;
+    job->target = jd->target_spp;
+    job->priority = jd->priority;
+    job->tag = jd->user_tag;
+    job->t_submit = s->now();
+    job->npix = (int64_t)o.width * o.height;
+    uint64_t h = lfh::hash_desc(*jd->scene);
+    auto it = s->scenes.find(h);
+    std::shared_ptr<lfh::SceneEntry> e;
+    if (it != s->scenes.end()) {
+        e = it->second;
+    } else {
+        e = std::make_shared<lfh::SceneEntry>();
+        e->hash = h;
+        std::string perr = lfh::pack_scene(*jd->scene, &e->ps);
+        if (!perr.empty()) {
+            job->state = -1;
+            if (job_id) *job_id = job->id;
+            s->jobs[job->id] = std::move(job);
+            return sfail(s, LFH_E_INVALID, "lfh_server_submit: " + perr);
+        }
+        e->bytes = lfh::packed_bytes(e->ps);
+    }
+    int64_t need = job->npix * (lfh::kAccumBytes + lfh::kPathBytes) + e->bytes;
+    if (job_id) *job_id = job->id;
+    if (need > s->budget) {
+        job->state = -1;
+        s->jobs[job->id] = std::move(job);
+        return sfail(s, LFH_E_BUDGET, "lfh_server_submit: job needs " + std::to_string(need) +
+                                          " device bytes, budget is " + std::to_string(s->budget));
+    }
+    s->scenes[h] = e;
+    job->scene = e;
+    e->refs++;
+    s->jobs[job->id] = std::move(job);
+    return LFH_OK;
+}
+
+int lfh_server_step(lfh_server* s, int32_t* active) {
+    if (!s) return sfail(nullptr, LFH_E_INVALID, "lfh_server_step: NULL server");
+    // -- admission: priority, then arrival; jobs that do not fit are skipped
+    std::vector<lfh::Job*> queued, running;
+    for (auto& kv : s->jobs) {
+        if (kv.second->state == 0) queued.push_back(kv.second.get());
+        if (kv.second->state == 1) running.push_back(kv.second.get());
+    }
+    std::stable_sort(queued.begin(), queued.end(),
+                     [](const lfh::Job* a, const lfh::Job* b) { return a->priority > b->priority; });
+    for (lfh::Job* j : queued) {
+        if ((int)running.size() >= s->max_jobs) break;
+        lfh::SceneEntry& e = *j->scene;
+        int64_t need = j->npix * (lfh::kAccumBytes + lfh::kPathBytes) + (e.resident ? 0 : e.bytes);
+        if (s->used + need > s->budget) evict_for(s, need);
+        if (s->used + need > s->budget) continue;
+        if (!e.resident) {
+            SCHK(s, lfh::upload_scene(e.ps, &e.ds, s->stream), "lfh_server_step: scene upload");
+            e.resident = true;
+            s->used += e.bytes;
+        }
+        SCHK(s, j->accum.alloc((size_t)j->npix * 16), "lfh_server_step: accumulation");
+        SCHK(s, j->comp.alloc((size_t)j->npix * 16), "lfh_server_step: accumulation");
+        SCHK(s, pt_clear4(j->accum.as<float4>(), j->npix, s->stream), "lfh_server_step: clear");
+        SCHK(s, pt_clear4(j->comp.as<float4>(), j->npix, s->stream), "lfh_server_step: clear");
+        j->charged = j->npix * (lfh::kAccumBytes + lfh::kPathBytes);
+        s->used += j->charged;
+        j->state = 1;
+        j->t_admit = s->now();
+        running.push_back(j);
+    }
+    // -- one pass for every running job, rays packed into shared queues
+    if (!running.empty()) {
+        std::sort(running.begin(), running.end(), [](const lfh::Job* a, const lfh::Job* b) { return a->id < b->id; });
+        int64_t total = 0;
+        int max_depth = 0;
+        std::vector<lfhd::JobDev> tab(running.size());
+        for (size_t k = 0; k < running.size(); ++k) {
+            lfh::Job* j = running[k];
+            lfhd::JobDev& d = tab[k];
+            std::memset(&d, 0, sizeof d);
+            d.sc = lfh::shade_view(j->scene->ps, j->scene->ds, j->opt);
+            lfhd::CamParams& c = d.cam;
+            std::memcpy(c.pos, j->cam.position, 12);
+            std::memcpy(c.right, j->cam.right, 12);
+            std::memcpy(c.up, j->cam.up, 12);
+            std::memcpy(c.fwd, j->cam.forward, 12);
+            c.fov = j->cam.fov;
+            c.focal_dist = j->cam.focal_dist;
+            c.aperture = j->cam.aperture;
+            c.width = j->opt.width;
+            c.height = j->opt.height;
+            c.tile_w = j->opt.width;
+            c.tile_h = j->opt.height;
+            c.tileX = 0;
+            c.tileY = 0;
+            c.frame = j->frame;
+            d.accum = j->accum.as<float4>();
+            d.comp = j->comp.as<float4>();
+            d.path_offset = (int)total;
+            d.kahan = j->opt.kahan;
+            total += j->npix;
+            max_depth = std::max(max_depth, j->opt.max_depth);
+            j->scene->last_use = s->now();
+        }
+        if (total > (1LL << 30)) return sfail(s, LFH_E_UNSUPPORTED, "lfh_server_step: too many pixels in flight");
+        if (s->pass.cap < total || s->pass.cap > 2 * total + (1 << 20)) {
+            SCHK(s, hipStreamSynchronize(s->stream), "lfh_server_step: sync");
+            SCHK(s, s->pass.alloc((int)total), "lfh_server_step: path buffers");
+        }
+        SCHK(s, hipMemcpyAsync(s->table.p, tab.data(), tab.size() * sizeof(lfhd::JobDev), hipMemcpyHostToDevice,
+                               s->stream),
+             "lfh_server_step: job table");
+        const lfhd::JobDev* jt = s->table.as<lfhd::JobDev>();
+        lfh::PassBufs& pb = s->pass;
+        PathBufs v = lfh::view(pb);
+        uint32_t* cnt = pb.counters.as<uint32_t>();
+        unsigned long long* work = pb.work.as<unsigned long long>();
+        SCHK(s, pt_reset(cnt, work, s->stream), "lfh_server_step: reset");
+        SCHK(s, pt_generate_multi(jt, (int)tab.size(), (int)total, v, pb.rq0.as<float4>(), cnt, s->stream),
+             "lfh_server_step: generate");
+        for (int depth = 0; depth < max_depth; ++depth) {
+            const int cur = depth & 1;
+            float4* q[2] = {pb.rq0.as<float4>(), pb.rq1.as<float4>()};
+            SCHK(s,
+                 pt_bounce_multi(jt, depth, pb.cap, q[cur], cnt + cur, q[cur ^ 1], cnt + (cur ^ 1), pb.hq.as<float4>(),
+                                 pb.sq.as<float4>(), cnt + 2, v, s->spill.as<uint32_t>(), work, s->errflag.as<int>(),
+                                 1.0f + 1.0f / 1024.0f, s->grid, s->stream),
+                 "lfh_server_step: bounce");
+        }
+        SCHK(s, pt_accumulate_multi(jt, (int)total, v, s->stream), "lfh_server_step: accumulate");
+        SCHK(s, hipStreamSynchronize(s->stream), "lfh_server_step: sync");
+        double t = s->now();
+        for (lfh::Job* j : running) {
+            j->spp_done++;
+            j->frame++;
+            if (j->spp_done >= j->target) {
+                j->state = 2;
+                j->t_done = t;
+                s->used -= j->npix * lfh::kPathBytes;  // path share back; accum held until release
+                j->charged = j->npix * lfh::kAccumBytes;
+            }
+        }
+    }
+    int32_t n = 0;
+    for (auto& kv : s->jobs)
+        if (kv.second->state == 0 || kv.second->state == 1) ++n;
+    if (active) *active = n;
+    return LFH_OK;
+}
+
+int lfh_server_status(lfh_server* s, int64_t id, lfh_job_status* st) {
+    if (!s || !st) return sfail(s, LFH_E_INVALID, "lfh_server_status: NULL argument");
+    auto it = s->jobs.find(id);
+    if (it == s->jobs.end()) return sfail(s, LFH_E_INVALID, "lfh_server_status: unknown job");
+    const lfh::Job& j = *it->second;
+    st->job_id = j.id;
+    st->state = j.state;
+    st->spp_done = j.spp_done;
+    st->t_submit = j.t_submit;
+    st->t_admit = j.t_admit;
+    st->t_done = j.t_done;
+    st->device_bytes = j.charged;
+    return LFH_OK;
+}
+
+int lfh_server_result(lfh_server* s, int64_t id, float* dst, int32_t w, int32_t h) {
+    if (!s || !dst) return sfail(s, LFH_E_INVALID, "lfh_server_result: NULL argument");
+    auto it = s->jobs.find(id);
+    if (it == s->jobs.end()) return sfail(s, LFH_E_INVALID, "lfh_server_result: unknown job");
+    lfh::Job& j = *it->second;
+    if (j.state != 2 && j.state != 1) return sfail(s, LFH_E_STATE, "lfh_server_result: job has no samples");
+    if (w != j.opt.width || h != j.opt.height) return sfail(s, LFH_E_INVALID, "lfh_server_result: size mismatch");
+    DevBuf out;
+    SCHK(s, out.alloc((size_t)j.npix * 12), "lfh_server_result: alloc");
+    SCHK(s, pt_output(j.accum.as<float4>(), j.comp.as<float4>(), j.npix, 1.0f / (float)std::max(1, j.spp_done),
+                      out.as<float>(), s->stream),
+         "lfh_server_result: kernel");
+    SCHK(s, hipMemcpyAsync(dst, out.p, (size_t)j.npix * 12, hipMemcpyDeviceToHost, s->stream), "lfh_server_result: copy");
+    SCHK(s, hipStreamSynchronize(s->stream), "lfh_server_result: sync");
+    return LFH_OK;
+}
+
+int lfh_server_release(lfh_server* s, int64_t id) {
+    if (!s) return sfail(nullptr, LFH_E_INVALID, "lfh_server_release: NULL server");
+    auto it = s->jobs.find(id);
+    if (it == s->jobs.end()) return sfail(s, LFH_E_INVALID, "lfh_server_release: unknown job");
+    lfh::Job& j = *it->second;
+    if (j.state == 1) return sfail(s, LFH_E_STATE, "lfh_server_release: job still running");
+    SCHK(s, hipStreamSynchronize(s->stream), "lfh_server_release: sync");
+    if (j.state == 2) s->used -= j.charged;
+    if (j.scene) j.scene->refs--;
+    s->jobs.erase(it);
+    return LFH_OK;
+}
+
+}  // extern "C"

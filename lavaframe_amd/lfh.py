@@ -559,3 +559,76 @@ class Context:
         rad = np.zeros((n, 3), np.float32)
         self._chk(self.lib.lfh_path_trace_rays(self.h, fptr(r), uptr(sd), n, fptr(rad)))
         return rad
+
+
+class Server:
+    """Render service (lfh_server_*): continuous batching of path-tracing
+    jobs against a device-memory budget."""
+
+    def __init__(self, budget_bytes: int = 0, max_jobs: int = 64, lib=None):
+        self.lib = lib or load()
+        h = C.c_void_p()
+        check(self.lib.lfh_server_create(int(budget_bytes), int(max_jobs), C.byref(h)), None, self.lib)
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.lfh_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, code):
+        return check(code, None, self.lib)
+
+    def submit(self, desc: SceneDesc, options: Options, camera: Camera, target_spp: int, priority: int = 0,
+               tag: int = 0) -> int:
+        jd = JobDesc()
+        jd.scene = C.pointer(desc)
+        jd.options = options
+        jd.camera = camera
+        jd.target_spp = target_spp
+        jd.priority = priority
+        jd.user_tag = tag
+        jid = C.c_int64()
+        code = self.lib.lfh_server_submit(self.h, C.byref(jd), C.byref(jid))
+        if code != LFH_OK:
+            raise LfhError(code, self.lib.lfh_last_error(None).decode())
+        return jid.value
+
+    def submit_scene(self, scene: "Scene", target_spp: int, **overrides) -> int:
+        o = scene.options()
+        for k, v in overrides.items():
+            setattr(o, k, v)
+        return self.submit(scene.desc(), o, scene.camera(), target_spp)
+
+    def step(self) -> int:
+        n = C.c_int32()
+        self._chk(self.lib.lfh_server_step(self.h, C.byref(n)))
+        return n.value
+
+    def status(self, job: int) -> JobStatus:
+        st = JobStatus()
+        self._chk(self.lib.lfh_server_status(self.h, job, C.byref(st)))
+        return st
+
+    def result(self, job: int, width: int, height: int) -> np.ndarray:
+        out = np.zeros((height, width, 3), np.float32)
+        self._chk(self.lib.lfh_server_result(self.h, job, fptr(out), width, height))
+        return out
+
+    def release(self, job: int):
+        self._chk(self.lib.lfh_server_release(self.h, job))
+
+    def clock(self) -> float:
+        return self.lib.lfh_server_clock(self.h)
+
+    def run_until_idle(self, max_steps: int = 1_000_000):
+        for _ in range(max_steps):
+            if self.step() == 0:
+                return
+        raise RuntimeError("server did not drain")

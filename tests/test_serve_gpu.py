@@ -1,0 +1,95 @@
+"""Render service: continuous batching of jobs whose rays share the wavefront
+queues.  A job's image is bit-identical alone, in a mix, and through the
+single-scene Renderer surface; admission respects the device budget."""
+import numpy as np
+import pytest
+
+from lavaframe_amd import lfh, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _jobs():
+    a = scenes.cornell(48, 40)
+    b = scenes.cornell(32, 32, max_depth=6)
+    c = scenes.soup(3000, seed=4)
+    c.set_camera((0.5, 0.5, -1.2), (0.5, 0.5, 0.5), 40.0)
+    c.add_light(lfh.quad_light((0.2, 1.5, 0.2), (0.8, 1.5, 0.2), (0.2, 1.5, 0.8), (4, 4, 4)))
+    c.set_options(width=40, height=24, max_depth=3, use_constant_bg=1, bg_color=(0.2, 0.3, 0.4))
+    c.build()
+    return [(a, 6), (b, 4), (c, 5)]
+
+
+def _render_alone_ctx(scene, spp):
+    ctx = lfh.Context.from_scene(scene, tile_width=0, tile_height=0)
+    o = scene.options()
+    ctx.options.tile_width, ctx.options.tile_height = o.width, o.height
+    cam = scene.camera()
+    for k in range(spp):
+        ctx.update(cam)
+        ctx.render()
+    return ctx.output_hdr()
+
+
+def test_alone_mix_and_renderer_agree_bitwise():
+    print("[serve] alone vs mix", flush=True)
+    jobs = _jobs()
+    alone = []
+    for sc, spp in jobs:
+        srv = lfh.Server(max_jobs=8)
+        jid = srv.submit_scene(sc, spp)
+        srv.run_until_idle()
+        o = sc.options()
+        alone.append(srv.result(jid, o.width, o.height))
+        st = srv.status(jid)
+        assert st.state == 2 and st.spp_done == spp
+    srv = lfh.Server(max_jobs=8)
+    ids = [srv.submit_scene(sc, spp) for sc, spp in jobs]
+    steps = 0
+    while srv.step():
+        steps += 1
+    assert steps <= max(spp for _, spp in jobs)  # all jobs progress together
+    for (sc, spp), jid, a in zip(jobs, ids, alone):
+        o = sc.options()
+        m = srv.result(jid, o.width, o.height)
+        assert np.array_equal(m.view(np.uint32), a.view(np.uint32))
+        r = _render_alone_ctx(sc, spp)
+        assert np.array_equal(r.view(np.uint32), a.view(np.uint32))
+        assert a.sum() > 0
+
+
+def test_budget_admission_and_rejection():
+    print("[serve] budget", flush=True)
+    sc = scenes.cornell(64, 64)
+    per_job = 64 * 64 * (32 + 16 * 17 + 4)
+    # room for the scene and about one job at a time
+    srv = lfh.Server(budget_bytes=int(1.5 * per_job) + 200_000, max_jobs=8)
+    ids = [srv.submit_scene(sc, 3) for _ in range(3)]
+    admits = set()
+    while srv.step():
+        running = [j for j in ids if srv.status(j).state == 1]
+        assert len(running) <= 1
+        admits.update(running)
+    sts = [srv.status(j) for j in ids]
+    assert all(s.state == 2 for s in sts)
+    t_admit = sorted(s.t_admit for s in sts)
+    assert t_admit[0] < t_admit[1] < t_admit[2]
+    # a job that can never fit is rejected at submission
+    big = scenes.cornell(1024, 1024)
+    with pytest.raises(lfh.LfhError) as e:
+        srv.submit_scene(big, 1)
+    assert e.value.code == -7
+
+
+def test_release_and_bad_job():
+    print("[serve] release", flush=True)
+    srv = lfh.Server(max_jobs=2)
+    sc = scenes.cornell(16, 16)
+    j = srv.submit_scene(sc, 2)
+    with pytest.raises(lfh.LfhError):
+        srv.release(12345)
+    srv.run_until_idle()
+    assert srv.status(j).state == 2
+    srv.release(j)
+    with pytest.raises(lfh.LfhError):
+        srv.status(j)
