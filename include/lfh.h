@@ -122,8 +122,12 @@ typedef struct lfh_options {
     float   bg_color[3];
     float   hdr_multiplier;
     int32_t kahan;                   /* 1: compensated accumulation (default) */
-    int32_t guiding;                 /* 0: plain BSDF sampling (the drop-in)  */
-    int32_t reserved[6];
+    int32_t guiding;                 /* 1: reserve learned-guide buffers (lfh_set_guide) */
+    int32_t estimator;               /* 0: the reference's MIS weights (drop-in);
+                                      * 1: consistent MIS, the BSDF strategy weighted by
+                                      *    its lobe-mixture pdf (disney.glsl:289-290) after
+                                      *    BSDF samples too (unbiased; used with guiding) */
+    int32_t reserved[5];
 } lfh_options;
 
 /* Camera uniforms (TiledRenderer.cpp:507-513); fov in radians. */
@@ -304,12 +308,27 @@ int  lfh_guide_forward(lfh_guide* g, const float* inputs9, int64_t n, float* out
  * loss = -sum_i w_i log p(d_i | x_i) / n; grads written as FP32 (n_params). */
 int  lfh_guide_loss_grad(lfh_guide* g, const float* inputs9, const float* dirs3,
                          const float* weights, int64_t n, float* loss, float* grads);
-/* One Adam step on a device-resident batch of radiance samples. */
+/* One Adam step (lr as created, beta 0.9 / 0.999, eps 1e-8) on a batch of at
+ * most 65536 (inputs, direction, weight) samples; loss as in lfh_guide_loss_grad. */
 int  lfh_guide_train_step(lfh_guide* g, const float* inputs9, const float* dirs3,
                           const float* weights, int64_t n, float* loss);
-/* Attach / detach a guide to a ctx (options.guiding must be set at lfh_create);
- * train_every: passes between online training steps (0 = frozen). */
+/* Number of Adam steps taken (explicit and online). */
+int  lfh_guide_steps(const lfh_guide* g, int64_t* steps);
+/* The mixture at each query: one direction drawn per query (seeds: 4 uint32
+ * pcg4d states per query, read only) with its mixture pdf; or the pdf of given
+ * directions. */
+int  lfh_guide_sample(lfh_guide* g, const float* inputs9, const uint32_t* seeds, int64_t n,
+                      float* dirs3, float* pdf);
+int  lfh_guide_pdf(lfh_guide* g, const float* inputs9, const float* dirs3, int64_t n, float* pdf);
+/* Attach (g != NULL) or detach a guide.  The ctx must have been created with
+ * options.guiding = 1 and options.estimator = 1: the shading stage then draws
+ * each BSDF direction from the guide or DisneySample with probability 1/2 and
+ * weights it by the mixture density.  train_every > 0: every train_every-th
+ * pass records training samples from a subset of its paths and takes one Adam
+ * step per 65536 of them after the pass; 0 = frozen guide. */
 int  lfh_set_guide(lfh_ctx* ctx, lfh_guide* g, int32_t train_every);
+/* Training samples collected by a ctx's online training so far. */
+int  lfh_guide_samples_seen(const lfh_ctx* ctx, int64_t* n);
 
 #ifdef __cplusplus
 }

@@ -50,8 +50,8 @@ int lfh_create(const lfh_scene_desc* desc, const lfh_options* opt, lfh_ctx** out
     *out = nullptr;
     if (opt->width <= 0 || opt->height <= 0 || opt->max_depth < 0)
         return ctx_fail(nullptr, LFH_E_INVALID, "lfh_create: bad resolution or depth");
-    if (opt->guiding)
-        return ctx_fail(nullptr, LFH_E_UNSUPPORTED, "lfh_create: guiding is attached with lfh_set_guide");
+    if (opt->estimator != 0 && opt->estimator != 1)
+        return ctx_fail(nullptr, LFH_E_INVALID, "lfh_create: estimator must be 0 or 1");
     lfh_ctx* c = new (std::nothrow) lfh_ctx;
     if (!c) return ctx_fail(nullptr, LFH_E_OOM, "lfh_create: out of host memory");
     c->opt = *opt;

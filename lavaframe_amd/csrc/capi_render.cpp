@@ -103,18 +103,22 @@ int render_tile(lfh_ctx* c) {
     unsigned long long* work = pb.work.as<unsigned long long>();
     RCHK(c, pt_reset(cnt, work, c->stream), "render: reset");
     RCHK(c, pt_generate(cp, lfh::view(pb), pb.rq0.as<float4>(), cnt + 0, c->stream), "render: generate");
-    GuideHook gh{0, 0, nullptr};
+    GuideHook gh{};
+    lfhd::GuideDev gd{};
+    if (guide_pass_begin(c, &gh, &gd) < 0) return rfail(c, LFH_E_HIP, "render: guide records");
     for (int depth = 0; depth < c->opt.max_depth; ++depth) {
         const int cur = depth & 1;
         float4* q[2] = {pb.rq0.as<float4>(), pb.rq1.as<float4>()};
         RCHK(c,
              pt_bounce(sc, depth, pb.cap, q[cur], cnt + cur, q[cur ^ 1], cnt + (cur ^ 1), pb.hq.as<float4>(),
                        pb.sq.as<float4>(), cnt + 2, lfh::view(pb), c->spill.as<uint32_t>(), work, c->errflag.as<int>(),
-                       c->cull, c->pt_grid, gh, c->stream),
+                       c->cull, c->pt_grid, gh, &gd, c->stream),
              "render: bounce");
     }
     RCHK(c, pt_accumulate(lfh::view(pb), cp, c->rs.accum.as<float4>(), c->rs.comp.as<float4>(), c->opt.kahan, c->stream),
          "render: accumulate");
+    int gr = guide_pass_end(c, gh);
+    if (gr != LFH_OK) return rfail(c, gr, std::string("render: ") + lfh_internal::thread_error());
     return LFH_OK;
 }
 
@@ -367,14 +371,14 @@ int lfh_path_trace_rays(lfh_ctx* c, const float* rays6, uint32_t* seeds, int64_t
     RCHK(c, pt_reset(cnt, work, c->stream), fn);
     RCHK(c, pt_generate_rays(dr, ds, (int)n, lfh::view(pb), pb.rq0.as<float4>(), cnt, c->stream), fn);
     lfhd::ShadeScene sc = lfh::shade_view(c->ps, c->ds, c->opt);
-    GuideHook gh{0, 0, nullptr};
+    GuideHook gh{};
     for (int depth = 0; depth < c->opt.max_depth; ++depth) {
         const int cur = depth & 1;
         float4* q[2] = {pb.rq0.as<float4>(), pb.rq1.as<float4>()};
         RCHK(c,
              pt_bounce(sc, depth, pb.cap, q[cur], cnt + cur, q[cur ^ 1], cnt + (cur ^ 1), pb.hq.as<float4>(),
                        pb.sq.as<float4>(), cnt + 2, lfh::view(pb), c->spill.as<uint32_t>(), work, c->errflag.as<int>(),
-                       c->cull, c->pt_grid, gh, c->stream),
+                       c->cull, c->pt_grid, gh, nullptr, c->stream),
              fn);
     }
     RCHK(c, pt_collect(lfh::view(pb), (int)n, dout, c->stream), fn);

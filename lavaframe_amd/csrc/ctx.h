@@ -117,6 +117,16 @@ struct PassBufs {
     }
 };
 
+// Per-context buffers of a learned guide attached with lfh_set_guide.
+struct GuideBufs {
+    DevBuf lobes;                  // kGuideLobeF4 float4 per hit-queue entry
+    DevBuf rec;                    // training records, 4 float4 per slot
+    DevBuf samples;                // compacted training samples, 4 float4 each
+    DevBuf block_counts, block_offsets, total, partial;
+    int stride = 1;                // record paths p with p % stride == phase
+    int nslots = 0;                // recorded paths x max_depth
+};
+
 hipError_t upload_scene(const PackedScene& ps, DeviceScene* ds, hipStream_t s);
 lfhd::TraceScene trace_view(const PackedScene& ps, const DeviceScene& ds);
 lfhd::ShadeScene shade_view(const PackedScene& ps, const DeviceScene& ds, const lfh_options& o);
@@ -150,4 +160,7 @@ struct lfh_ctx {
     // guiding
     struct lfh_guide* guide = nullptr;
     int guide_train_every = 0;
+    int guide_pass = 0;            // passes rendered with the guide attached
+    int64_t guide_samples = 0;     // training samples collected so far
+    lfh::GuideBufs gbuf;
 };

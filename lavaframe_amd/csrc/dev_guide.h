@@ -1,0 +1,186 @@
+// Learned path guide: device model shared by the MLP kernels and the guided
+// shading stage.  The guide has no counterpart in the reference renderer
+// (LavaFrame samples the BSDF only, pathtrace.glsl:268-291); it adds a second
+// sampling strategy, one-sample-MIS mixed with the BSDF.
+//
+//   encoding  64 features of (x = scene-normalised position in [0,1]^3, n, wo):
+//             [0,36)  sin/cos(2^f pi x_a), f = 0..5     index (a*6+f)*2 + {sin,cos}
+//             [36,48) sin/cos(2^f pi n_a), f = 0..1     36 + (a*2+f)*2 + {sin,cos}
+//             [48,60) sin/cos(2^f pi wo_a), f = 0..1    48 + (a*2+f)*2 + {sin,cos}
+//             [60,63) wo, 63: 1
+//   MLP       64 -> 64 -> 64 -> 64 -> 40, ReLU after the hidden layers,
+//             y = x W + b (W [in][out]); BF16 operands, FP32 accumulation
+//   lobes     k = 0..7 from outputs 5k..5k+4: mu = normalize(r), kappa =
+//             exp(clamp(s, -3, 7)), pi = softmax(a)
+//   pdf       sum_k pi_k kappa_k / (2 pi (1 - e^{-2 kappa_k})) e^{kappa_k (mu_k.d - 1)}
+#pragma once
+#include "dev_common.h"
+
+namespace lfhd {
+
+constexpr int kGuideIn = 64;
+constexpr int kGuideHid = 64;
+constexpr int kGuideLobes = 8;
+constexpr int kGuideOut = 40;
+constexpr int kGuideLayers = 4;
+// flat FP32 parameter vector: per layer W [in][out] row-major, then b [out]
+constexpr int kGuideParams = 3 * (64 * 64 + 64) + 64 * 40 + 40;  // 15080
+constexpr float kLogKMin = -3.0f;
+constexpr float kLogKMax = 7.0f;
+constexpr float kGuideFrac = 0.5f;   // probability of drawing from the guide
+constexpr int kGuideLobeF4 = 10;     // processed lobes per query: 8 x (mu, kappa), 2 x pi[4]
+
+LFH_HD int guide_layer_out(int l) { return l == 3 ? kGuideOut : kGuideHid; }
+LFH_HD int guide_layer_offset(int l) { return l * (64 * 64 + 64); }
+
+// Device view of the guide's weights.  wf: per layer [64 out][64 in] BF16 (the
+// forward product's B operand, zero rows past the layer's outputs); wb: per
+// layer [64 in][64 out] BF16 (the backward product's B operand); bias: per
+// layer 64 FP32 (zero-padded).
+struct GuideDev {
+    const uint16_t* wf;
+    const uint16_t* wb;
+    const float* bias;
+    float lo[3];
+    float inv_ext[3];
+};
+
+LFH_HD float guide_norm_coord(float p, float lo, float inv_ext) {
+    float x = (p - lo) * inv_ext;
+    return x < 0.0f ? 0.0f : (x > 1.0f ? 1.0f : x);
+}
+
+#if defined(__HIP__)
+// Feature f of the encoding (x already normalised).
+LFH_DEV float guide_feature(int f, V3 x, V3 n, V3 wo) {
+    if (f >= 63) return 1.0f;
+    if (f >= 60) return f == 60 ? wo.x : (f == 61 ? wo.y : wo.z);
+    float val;
+    int a, fr, sc;
+    if (f < 36) {
+        a = f / 12;
+        fr = (f % 12) >> 1;
+        sc = f & 1;
+        val = a == 0 ? x.x : (a == 1 ? x.y : x.z);
+    } else {
+        const int g = f < 48 ? f - 36 : f - 48;
+        const V3 s = f < 48 ? n : wo;
+        a = g >> 2;
+        fr = (g >> 1) & 1;
+        sc = g & 1;
+        val = a == 0 ? s.x : (a == 1 ? s.y : s.z);
+    }
+    const float ang = (float)(1 << fr) * kPi * val;
+    return sc ? cosf(ang) : sinf(ang);
+}
+
+LFH_DEV uint16_t to_bf16(float v) {
+    uint32_t u = __float_as_uint(v);
+    u += 0x7FFFu + ((u >> 16) & 1u);   // round to nearest even
+    return (uint16_t)(u >> 16);
+}
+LFH_DEV float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// 40 raw outputs -> 10 float4: lobes (mu, kappa) and the mixture weights.
+LFH_DEV void guide_lobes(const float* o, float4* out) {
+    float a[kGuideLobes];
+    float amax = -3.0e38f;
+#pragma unroll
+    for (int k = 0; k < kGuideLobes; ++k) {
+        a[k] = o[5 * k + 4];
+        amax = fmaxf(amax, a[k]);
+    }
+    float sum = 0.0f;
+#pragma unroll
+    for (int k = 0; k < kGuideLobes; ++k) {
+        a[k] = expf(a[k] - amax);
+        sum += a[k];
+    }
+    const float inv = 1.0f / sum;
+#pragma unroll
+    for (int k = 0; k < kGuideLobes; ++k) {
+        float rx = o[5 * k], ry = o[5 * k + 1], rz = o[5 * k + 2];
+        float rn = sqrtf(rx * rx + ry * ry + rz * rz);
+        float ir = rn > 1e-12f ? 1.0f / rn : 0.0f;
+        if (rn <= 1e-12f) rz = 1.0f, ir = 1.0f;
+        float s = fminf(fmaxf(o[5 * k + 3], kLogKMin), kLogKMax);
+        out[k] = make_float4(rx * ir, ry * ir, rz * ir, expf(s));
+    }
+    out[8] = make_float4(a[0] * inv, a[1] * inv, a[2] * inv, a[3] * inv);
+    out[9] = make_float4(a[4] * inv, a[5] * inv, a[6] * inv, a[7] * inv);
+}
+
+// Stores master parameter j (flat layout) into the BF16 operand copies:
+// wf[layer][out][in], wb[layer][in][out], or the FP32 bias.
+LFH_DEV void guide_store_param(uint16_t* wf, uint16_t* wb, float* bias, int j, float w) {
+    const int layer = j < 3 * (64 * 64 + 64) ? j / (64 * 64 + 64) : 3;
+    const int r = j - guide_layer_offset(layer);
+    const int nout = guide_layer_out(layer);
+    if (r < 64 * nout) {
+        const int in = r / nout, out = r - in * nout;
+        const uint16_t h = to_bf16(w);
+        wf[layer * 4096 + out * 64 + in] = h;
+        wb[layer * 4096 + in * 64 + out] = h;
+    } else {
+        bias[layer * 64 + (r - 64 * nout)] = w;
+    }
+}
+
+LFH_DEV float vmf_norm(float kappa) { return kappa / (kTwoPi * -expm1f(-2.0f * kappa)); }
+
+LFH_DEV float guide_pdf(const float4* __restrict__ lob, V3 d) {
+    const float4 pa = lob[8], pb = lob[9];
+    const float pis[8] = {pa.x, pa.y, pa.z, pa.w, pb.x, pb.y, pb.z, pb.w};
+    float p = 0.0f;
+#pragma unroll
+    for (int k = 0; k < kGuideLobes; ++k) {
+        const float4 l = lob[k];
+        const float c = l.x * d.x + l.y * d.y + l.z * d.z;
+        p += pis[k] * vmf_norm(l.w) * expf(l.w * (c - 1.0f));
+    }
+    return p;
+}
+
+// Orthonormal basis around a unit vector (Duff et al. 2017, "Building an
+// orthonormal basis, revisited").
+LFH_DEV void guide_frame(V3 n, V3& t, V3& b) {
+    const float sgn = n.z >= 0.0f ? 1.0f : -1.0f;
+    const float a = -1.0f / (sgn + n.z);
+    const float bb = n.x * n.y * a;
+    t = v3(1.0f + sgn * n.x * n.x * a, sgn * bb, -sgn * n.x);
+    b = v3(bb, sgn + n.y * n.y * a, -n.y);
+}
+
+// Draws a lobe by its weight (u0), then a direction from that vMF lobe by the
+// inverse CDF of cos(theta) (Jakob 2012, "Numerically stable sampling of the
+// von Mises Fisher distribution on S2"): w = 1 + log(u1 + (1-u1) e^{-2 kappa}) / kappa,
+// phi = 2 pi u2.
+LFH_DEV V3 guide_sample(const float4* __restrict__ lob, float u0, float u1, float u2) {
+    const float4 pa = lob[8], pb = lob[9];
+    const float pis[8] = {pa.x, pa.y, pa.z, pa.w, pb.x, pb.y, pb.z, pb.w};
+    int k = kGuideLobes - 1;
+    float cum = 0.0f;
+#pragma unroll
+    for (int j = 0; j < kGuideLobes - 1; ++j) {
+        cum += pis[j];
+        if (u0 < cum && k == kGuideLobes - 1) k = j;
+    }
+    float4 l = lob[0];
+#pragma unroll
+    for (int j = 1; j < kGuideLobes; ++j)
+        if (j == k) l = lob[j];
+    const float kappa = l.w;
+    float w = 1.0f + logf(u1 + (1.0f - u1) * expf(-2.0f * kappa)) / kappa;
+    w = fminf(fmaxf(w, -1.0f), 1.0f);
+    const float s = sqrtf(fmaxf(0.0f, 1.0f - w * w));
+    const V3 mu = v3(l.x, l.y, l.z);
+    V3 t, b;
+    guide_frame(mu, t, b);
+    const float ph = kTwoPi * u2;
+    const float cp = cosf(ph), sp = sinf(ph);
+    return v3(w * mu.x + s * (cp * t.x + sp * b.x), w * mu.y + s * (cp * t.y + sp * b.y),
+              w * mu.z + s * (cp * t.z + sp * b.z));
+}
+#endif  // __HIP__
+
+}  // namespace lfhd

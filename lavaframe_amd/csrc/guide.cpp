@@ -1,0 +1,414 @@
+// Learned path guide: lfh_guide_* entry points, online training from render
+// passes, and the per-pass hooks of the renderer.  The guide has no
+// counterpart in the reference renderer; it adds a second sampling strategy
+// (BF16 MFMA MLP -> 8 vMF lobes) that the shading stage mixes 50/50 with
+// DisneySample under the consistent one-sample MIS estimator.
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/lfh.h"
+#include "capi_common.h"
+#include "ctx.h"
+#include "guide_kernels.h"
+#include "pt_kernels.h"
+#include "render.h"
+
+using lfhd::kGuideLobeF4;
+using lfhd::kGuideOut;
+using lfhd::kGuideParams;
+
+namespace {
+constexpr int kBatch = 65536;             // samples per optimisation step
+constexpr int kMaxSamples = 4 * kBatch;   // samples kept from one training pass
+constexpr int kRecordedPaths = 32768;     // target number of recorded paths per pass
+}  // namespace
+
+struct lfh_guide {
+    std::string err;
+    hipStream_t stream = nullptr;
+    lfh::DevBuf master, m, v, wf, wb, bias;
+    lfh::DevBuf partial, samples, count, grad, loss, in, aux, out;
+    float lo[3], hi[3];
+    float lr = 1e-3f;
+    int64_t steps = 0;   // Adam steps taken
+
+    lfhd::GuideDev dev() const {
+        lfhd::GuideDev g;
+        g.wf = wf.as<const uint16_t>();
+        g.wb = wb.as<const uint16_t>();
+        g.bias = bias.as<const float>();
+        for (int a = 0; a < 3; ++a) {
+            g.lo[a] = lo[a];
+            float e = hi[a] - lo[a];
+            g.inv_ext[a] = 1.0f / (e > 1e-6f ? e : 1e-6f);
+        }
+        return g;
+    }
+    lfhd::GuideAdam adam(bool step) {
+        lfhd::GuideAdam a;
+        a.master = master.as<float>();
+        a.m = m.as<float>();
+        a.v = v.as<float>();
+        a.wf = wf.as<uint16_t>();
+        a.wb = wb.as<uint16_t>();
+        a.bias = bias.as<float>();
+        a.lr = lr;
+        a.b1 = 0.9f;
+        a.b2 = 0.999f;
+        a.eps = 1e-8f;
+        const double t = (double)(step ? steps + 1 : 1);
+        a.c1 = (float)(1.0 / (1.0 - std::pow(0.9, t)));
+        a.c2 = (float)(1.0 / (1.0 - std::pow(0.999, t)));
+        return a;
+    }
+};
+
+namespace {
+
+int gfail(lfh_guide* g, int code, const std::string& msg) {
+    if (g) g->err = msg;
+    lfh_internal::set_thread_error(msg);
+    return code;
+}
+int ghip(lfh_guide* g, hipError_t e, const char* where) {
+    return gfail(g, e == hipErrorOutOfMemory ? LFH_E_OOM : LFH_E_HIP,
+                 std::string(where) + ": " + hipGetErrorString(e));
+}
+#define GCHK(g, expr, where)                          \
+    do {                                              \
+        hipError_t _e = (expr);                       \
+        if (_e != hipSuccess) return ghip(g, _e, where); \
+    } while (0)
+
+hipError_t ensure(lfh::DevBuf& b, size_t bytes) { return b.bytes >= bytes ? hipSuccess : b.alloc(bytes); }
+
+uint64_t splitmix(uint64_t& s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// He-uniform weights (bound sqrt(6 / fan_in)), output layer scaled by 0.1,
+// zero biases.
+std::vector<float> init_params(uint64_t seed) {
+    std::vector<float> p(kGuideParams, 0.0f);
+    uint64_t s = seed;
+    for (int l = 0; l < lfhd::kGuideLayers; ++l) {
+        const int nout = lfhd::guide_layer_out(l), off = lfhd::guide_layer_offset(l);
+        const float bound = std::sqrt(6.0f / 64.0f) * (l == 3 ? 0.1f : 1.0f);
+        for (int i = 0; i < 64 * nout; ++i) {
+            const double u = (double)(splitmix(s) >> 11) * (1.0 / 9007199254740992.0);
+            p[off + i] = (float)((2.0 * u - 1.0) * bound);
+        }
+    }
+    return p;
+}
+
+// Uploads params as the master copy, resets Adam, rebuilds the BF16 copies.
+int load_params(lfh_guide* g, const float* p) {
+    GCHK(g, hipMemcpyAsync(g->master.p, p, kGuideParams * sizeof(float), hipMemcpyHostToDevice, g->stream),
+         "guide: upload");
+    GCHK(g, hipMemsetAsync(g->m.p, 0, kGuideParams * sizeof(float), g->stream), "guide: reset");
+    GCHK(g, hipMemsetAsync(g->v.p, 0, kGuideParams * sizeof(float), g->stream), "guide: reset");
+    GCHK(g, lfhd::guide_refresh(g->master.as<float>(), g->wf.as<uint16_t>(), g->wb.as<uint16_t>(),
+                                g->bias.as<float>(), g->stream),
+         "guide: refresh");
+    GCHK(g, hipStreamSynchronize(g->stream), "guide: sync");
+    g->steps = 0;
+    return LFH_OK;
+}
+
+// Host samples (pos, n, wo) + dirs + weights -> device sample records.
+int upload_batch(lfh_guide* g, const float* in9, const float* dirs3, const float* w, int64_t n) {
+    std::vector<float> h((size_t)n * 16);
+    for (int64_t i = 0; i < n; ++i) {
+        float* r = &h[(size_t)i * 16];
+        const float* x = in9 + i * 9;
+        r[0] = x[0], r[1] = x[1], r[2] = x[2], r[3] = w[i];
+        r[4] = x[3], r[5] = x[4], r[6] = x[5], r[7] = 0.0f;
+        r[8] = x[6], r[9] = x[7], r[10] = x[8], r[11] = 0.0f;
+        r[12] = dirs3[i * 3], r[13] = dirs3[i * 3 + 1], r[14] = dirs3[i * 3 + 2], r[15] = 0.0f;
+    }
+    GCHK(g, ensure(g->samples, h.size() * sizeof(float)), "guide: alloc");
+    GCHK(g, hipMemcpyAsync(g->samples.p, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, g->stream),
+         "guide: upload");
+    const int cnt = (int)n;
+    GCHK(g, hipMemcpyAsync(g->count.p, &cnt, sizeof(int), hipMemcpyHostToDevice, g->stream), "guide: upload");
+    GCHK(g, hipStreamSynchronize(g->stream), "guide: sync");
+    return LFH_OK;
+}
+
+int check_batch(lfh_guide* g, const char* fn, const void* a, const void* b, int64_t n) {
+    if (!g) return gfail(nullptr, LFH_E_INVALID, std::string(fn) + ": NULL guide");
+    if (n < 0 || n > (int64_t)1 << 24) return gfail(g, LFH_E_INVALID, std::string(fn) + ": bad batch size");
+    if (n > 0 && (!a || !b)) return gfail(g, LFH_E_INVALID, std::string(fn) + ": NULL argument");
+    return LFH_OK;
+}
+
+// Forward of host queries into g->aux (lobes) and/or g->out (raw outputs).
+int forward_host(lfh_guide* g, const float* in9, int64_t n, bool raw, bool lobes) {
+    GCHK(g, ensure(g->in, (size_t)n * 9 * sizeof(float)), "guide: alloc");
+    if (raw) GCHK(g, ensure(g->out, (size_t)n * kGuideOut * sizeof(float)), "guide: alloc");
+    if (lobes) GCHK(g, ensure(g->aux, (size_t)n * kGuideLobeF4 * sizeof(float4)), "guide: alloc");
+    GCHK(g, hipMemcpyAsync(g->in.p, in9, (size_t)n * 9 * sizeof(float), hipMemcpyHostToDevice, g->stream),
+         "guide: upload");
+    GCHK(g, lfhd::guide_forward_host(g->dev(), g->in.as<float>(), (int)n, raw ? g->out.as<float>() : nullptr,
+                                     lobes ? g->aux.as<float4>() : nullptr, g->stream),
+         "guide: forward");
+    return LFH_OK;
+}
+
+int run_batch(lfh_guide* g, int64_t n, bool step, float* loss, float* grads) {
+    const int batch = (int)(n > 0 ? n : 1);
+    GCHK(g, ensure(g->partial, lfhd::guide_partial_floats(batch) * sizeof(float)), "guide: alloc");
+    lfhd::GuideAdam ad = g->adam(step);
+    GCHK(g, lfhd::guide_train(g->dev(), g->samples.as<float4>(), g->count.as<int>(), 0, batch,
+                              g->partial.as<float>(), ad, step ? nullptr : g->grad.as<float>(), g->loss.as<float>(),
+                              g->stream),
+         "guide: train");
+    if (step && n > 0) g->steps++;
+    if (grads)
+        GCHK(g, hipMemcpyAsync(grads, g->grad.p, kGuideParams * sizeof(float), hipMemcpyDeviceToHost, g->stream),
+             "guide: download");
+    if (loss) GCHK(g, hipMemcpyAsync(loss, g->loss.p, sizeof(float), hipMemcpyDeviceToHost, g->stream), "guide: download");
+    GCHK(g, hipStreamSynchronize(g->stream), "guide: sync");
+    return LFH_OK;
+}
+
+}  // namespace
+
+namespace lfh {
+
+// Per-pass hooks of the renderer ------------------------------------------------------
+int guide_pass_begin(lfh_ctx* c, GuideHook* gh, lfhd::GuideDev* gd) {
+    *gh = GuideHook{};
+    lfh_guide* g = c->guide;
+    if (!g) return 0;
+    GuideBufs& b = c->gbuf;
+    gh->enabled = 1;
+    gh->lobes = b.lobes.as<const float4>();
+    gh->max_depth = c->opt.max_depth;
+    *gd = g->dev();
+    const bool train = c->guide_train_every > 0 && (c->guide_pass % c->guide_train_every) == 0;
+    if (train && c->opt.max_depth > 0) {
+        gh->train = 1;
+        gh->stride = b.stride;
+        gh->phase = (c->guide_pass / c->guide_train_every) % b.stride;
+        gh->rec = b.rec.as<float4>();
+        if (pt_clear4(b.rec.as<float4>(), (int64_t)b.nslots * 4, c->stream) != hipSuccess) return -1;
+    }
+    return 1;
+}
+
+int guide_pass_end(lfh_ctx* c, const GuideHook& gh) {
+    lfh_guide* g = c->guide;
+    c->guide_pass++;
+    if (!g || !gh.train) return LFH_OK;
+    GuideBufs& b = c->gbuf;
+    hipError_t e = lfhd::guide_collect(b.rec.as<float4>(), b.nslots, c->opt.max_depth, gh.stride, gh.phase,
+                                       c->pass.rad.as<float4>(), c->rs.npaths, b.block_counts.as<int>(),
+                                       b.block_offsets.as<int>(), b.total.as<int>(), b.samples.as<float4>(),
+                                       kMaxSamples, c->stream);
+    int total = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&total, b.total.p, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return ghip(g, e, "guide: collect");
+    c->guide_samples += total;
+    for (int start = 0; start < total; start += kBatch) {
+        lfhd::GuideAdam ad = g->adam(true);
+        e = lfhd::guide_train(g->dev(), b.samples.as<float4>(), b.total.as<int>(), start, kBatch,
+                              b.partial.as<float>(), ad, nullptr, nullptr, c->stream);
+        if (e != hipSuccess) return ghip(g, e, "guide: train");
+        g->steps++;
+    }
+    return LFH_OK;
+}
+
+}  // namespace lfh
+
+extern "C" {
+
+int lfh_guide_create(const float* scene_min3, const float* scene_max3, uint64_t seed, float learning_rate,
+                     lfh_guide** out) {
+    if (!scene_min3 || !scene_max3 || !out) return gfail(nullptr, LFH_E_INVALID, "lfh_guide_create: NULL argument");
+    *out = nullptr;
+    if (!(learning_rate > 0.0f)) return gfail(nullptr, LFH_E_INVALID, "lfh_guide_create: learning rate must be > 0");
+    lfh_guide* g = new (std::nothrow) lfh_guide;
+    if (!g) return gfail(nullptr, LFH_E_OOM, "lfh_guide_create: out of host memory");
+    for (int a = 0; a < 3; ++a) {
+        g->lo[a] = scene_min3[a];
+        g->hi[a] = scene_max3[a];
+    }
+    g->lr = learning_rate;
+    hipError_t e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = g->master.alloc(kGuideParams * sizeof(float));
+    if (e == hipSuccess) e = g->m.alloc(kGuideParams * sizeof(float));
+    if (e == hipSuccess) e = g->v.alloc(kGuideParams * sizeof(float));
+    if (e == hipSuccess) e = g->wf.alloc(4 * 4096 * sizeof(uint16_t));
+    if (e == hipSuccess) e = g->wb.alloc(4 * 4096 * sizeof(uint16_t));
+    if (e == hipSuccess) e = g->bias.alloc(4 * 64 * sizeof(float));
+    if (e == hipSuccess) e = g->count.alloc(64);
+    if (e == hipSuccess) e = g->grad.alloc((kGuideParams + 1) * sizeof(float));
+    if (e == hipSuccess) e = g->loss.alloc(64);
+    if (e == hipSuccess) e = hipMemsetAsync(g->wf.p, 0, g->wf.bytes, g->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(g->wb.p, 0, g->wb.bytes, g->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(g->bias.p, 0, g->bias.bytes, g->stream);
+    if (e != hipSuccess) {
+        int r = ghip(nullptr, e, "lfh_guide_create");
+        lfh_guide_destroy(g);
+        return r;
+    }
+    std::vector<float> p = init_params(seed);
+    int r = load_params(g, p.data());
+    if (r != LFH_OK) {
+        lfh_guide_destroy(g);
+        return r;
+    }
+    *out = g;
+    return LFH_OK;
+}
+
+void lfh_guide_destroy(lfh_guide* g) {
+    if (!g) return;
+    if (g->stream) {
+        (void)hipStreamSynchronize(g->stream);
+        (void)hipStreamDestroy(g->stream);
+    }
+    delete g;
+}
+
+int lfh_guide_get_params(lfh_guide* g, float* params, int64_t* n_params) {
+    if (!g || !n_params) return gfail(g, LFH_E_INVALID, "lfh_guide_get_params: NULL argument");
+    if (!params) {
+        *n_params = kGuideParams;
+        return LFH_OK;
+    }
+    if (*n_params < kGuideParams) return gfail(g, LFH_E_INVALID, "lfh_guide_get_params: buffer too small");
+    *n_params = kGuideParams;
+    GCHK(g, hipMemcpyAsync(params, g->master.p, kGuideParams * sizeof(float), hipMemcpyDeviceToHost, g->stream),
+         "lfh_guide_get_params");
+    GCHK(g, hipStreamSynchronize(g->stream), "lfh_guide_get_params");
+    return LFH_OK;
+}
+
+int lfh_guide_set_params(lfh_guide* g, const float* params, int64_t n_params) {
+    if (!g || !params) return gfail(g, LFH_E_INVALID, "lfh_guide_set_params: NULL argument");
+    if (n_params != kGuideParams) return gfail(g, LFH_E_INVALID, "lfh_guide_set_params: wrong parameter count");
+    return load_params(g, params);
+}
+
+int lfh_guide_forward(lfh_guide* g, const float* inputs9, int64_t n, float* out40) {
+    int r = check_batch(g, "lfh_guide_forward", inputs9, out40, n);
+    if (r != LFH_OK || n == 0) return r;
+    if ((r = forward_host(g, inputs9, n, true, false)) != LFH_OK) return r;
+    GCHK(g, hipMemcpyAsync(out40, g->out.p, (size_t)n * kGuideOut * sizeof(float), hipMemcpyDeviceToHost, g->stream),
+         "lfh_guide_forward");
+    GCHK(g, hipStreamSynchronize(g->stream), "lfh_guide_forward");
+    return LFH_OK;
+}
+
+int lfh_guide_sample(lfh_guide* g, const float* inputs9, const uint32_t* seeds, int64_t n, float* dirs3,
+                     float* pdf) {
+    int r = check_batch(g, "lfh_guide_sample", inputs9, seeds, n);
+    if (r != LFH_OK || n == 0) return r;
+    if (!dirs3 || !pdf) return gfail(g, LFH_E_INVALID, "lfh_guide_sample: NULL argument");
+    if ((r = forward_host(g, inputs9, n, false, true)) != LFH_OK) return r;
+    GCHK(g, ensure(g->out, (size_t)n * 16 * sizeof(float)), "lfh_guide_sample");
+    float* d_seeds = g->out.as<float>();
+    float* d_dirs = d_seeds + (size_t)n * 4;
+    float* d_pdf = d_seeds + (size_t)n * 7;
+    GCHK(g, hipMemcpyAsync(d_seeds, seeds, (size_t)n * 16, hipMemcpyHostToDevice, g->stream), "lfh_guide_sample");
+    GCHK(g, lfhd::guide_sample(g->aux.as<float4>(), reinterpret_cast<const uint4*>(d_seeds), (int)n, d_dirs, d_pdf,
+                               g->stream),
+         "lfh_guide_sample");
+    GCHK(g, hipMemcpyAsync(dirs3, d_dirs, (size_t)n * 12, hipMemcpyDeviceToHost, g->stream), "lfh_guide_sample");
+    GCHK(g, hipMemcpyAsync(pdf, d_pdf, (size_t)n * 4, hipMemcpyDeviceToHost, g->stream), "lfh_guide_sample");
+    GCHK(g, hipStreamSynchronize(g->stream), "lfh_guide_sample");
+    return LFH_OK;
+}
+
+int lfh_guide_pdf(lfh_guide* g, const float* inputs9, const float* dirs3, int64_t n, float* pdf) {
+    int r = check_batch(g, "lfh_guide_pdf", inputs9, dirs3, n);
+    if (r != LFH_OK || n == 0) return r;
+    if (!pdf) return gfail(g, LFH_E_INVALID, "lfh_guide_pdf: NULL argument");
+    if ((r = forward_host(g, inputs9, n, false, true)) != LFH_OK) return r;
+    GCHK(g, ensure(g->out, (size_t)n * 4 * sizeof(float)), "lfh_guide_pdf");
+    float* d_dirs = g->out.as<float>();
+    float* d_pdf = d_dirs + (size_t)n * 3;
+    GCHK(g, hipMemcpyAsync(d_dirs, dirs3, (size_t)n * 12, hipMemcpyHostToDevice, g->stream), "lfh_guide_pdf");
+    GCHK(g, lfhd::guide_pdf(g->aux.as<float4>(), d_dirs, (int)n, d_pdf, g->stream), "lfh_guide_pdf");
+    GCHK(g, hipMemcpyAsync(pdf, d_pdf, (size_t)n * 4, hipMemcpyDeviceToHost, g->stream), "lfh_guide_pdf");
+    GCHK(g, hipStreamSynchronize(g->stream), "lfh_guide_pdf");
+    return LFH_OK;
+}
+
+int lfh_guide_loss_grad(lfh_guide* g, const float* inputs9, const float* dirs3, const float* weights, int64_t n,
+                        float* loss, float* grads) {
+    int r = check_batch(g, "lfh_guide_loss_grad", inputs9, dirs3, n);
+    if (r != LFH_OK) return r;
+    if (n > 0 && !weights) return gfail(g, LFH_E_INVALID, "lfh_guide_loss_grad: NULL argument");
+    if (n > kBatch) return gfail(g, LFH_E_INVALID, "lfh_guide_loss_grad: at most 65536 samples per batch");
+    if ((r = upload_batch(g, inputs9, dirs3, weights, n)) != LFH_OK) return r;
+    return run_batch(g, n, false, loss, grads);
+}
+
+int lfh_guide_train_step(lfh_guide* g, const float* inputs9, const float* dirs3, const float* weights, int64_t n,
+                         float* loss) {
+    int r = check_batch(g, "lfh_guide_train_step", inputs9, dirs3, n);
+    if (r != LFH_OK) return r;
+    if (n > 0 && !weights) return gfail(g, LFH_E_INVALID, "lfh_guide_train_step: NULL argument");
+    if (n > kBatch) return gfail(g, LFH_E_INVALID, "lfh_guide_train_step: at most 65536 samples per batch");
+    if (n == 0) return LFH_OK;
+    if ((r = upload_batch(g, inputs9, dirs3, weights, n)) != LFH_OK) return r;
+    return run_batch(g, n, true, loss, nullptr);
+}
+
+int lfh_guide_steps(const lfh_guide* g, int64_t* steps) {
+    if (!g || !steps) return gfail(nullptr, LFH_E_INVALID, "lfh_guide_steps: NULL argument");
+    *steps = g->steps;
+    return LFH_OK;
+}
+
+int lfh_set_guide(lfh_ctx* c, lfh_guide* g, int32_t train_every) {
+    if (!c) return gfail(nullptr, LFH_E_INVALID, "lfh_set_guide: NULL ctx");
+    if (train_every < 0) return gfail(g, LFH_E_INVALID, "lfh_set_guide: train_every must be >= 0");
+    if (g && !c->opt.guiding)
+        return gfail(g, LFH_E_STATE, "lfh_set_guide: the context was created without options.guiding");
+    if (g && c->opt.estimator != 1)
+        return gfail(g, LFH_E_STATE, "lfh_set_guide: guided sampling needs options.estimator = 1");
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return gfail(g, LFH_E_HIP, "lfh_set_guide: sync");
+    c->guide = g;
+    c->guide_train_every = train_every;
+    if (!g) return LFH_OK;
+    lfh::GuideBufs& b = c->gbuf;
+    const int npaths = c->rs.npaths;
+    b.stride = (npaths + kRecordedPaths - 1) / kRecordedPaths;
+    if (b.stride < 1) b.stride = 1;
+    const int recorded = (npaths + b.stride - 1) / b.stride;
+    b.nslots = recorded * (c->opt.max_depth > 0 ? c->opt.max_depth : 1);
+    const int nblocks = lfhd::guide_collect_blocks(b.nslots);
+    hipError_t e = ensure(b.lobes, (size_t)c->pass.cap * kGuideLobeF4 * sizeof(float4));
+    if (e == hipSuccess) e = ensure(b.rec, (size_t)b.nslots * 4 * sizeof(float4));
+    if (e == hipSuccess) e = ensure(b.samples, (size_t)kMaxSamples * 4 * sizeof(float4));
+    if (e == hipSuccess) e = ensure(b.block_counts, (size_t)(nblocks + 1) * sizeof(int));
+    if (e == hipSuccess) e = ensure(b.block_offsets, (size_t)(nblocks + 1) * sizeof(int));
+    if (e == hipSuccess) e = ensure(b.total, 64);
+    if (e == hipSuccess) e = ensure(b.partial, lfhd::guide_partial_floats(kBatch) * sizeof(float));
+    if (e != hipSuccess) {
+        c->guide = nullptr;
+        return ghip(g, e, "lfh_set_guide");
+    }
+    return LFH_OK;
+}
+
+int lfh_guide_samples_seen(const lfh_ctx* c, int64_t* n) {
+    if (!c || !n) return gfail(nullptr, LFH_E_INVALID, "lfh_guide_samples_seen: NULL argument");
+    *n = c->guide_samples;
+    return LFH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,635 @@
+// Learned path guide on gfx950: BF16 MFMA (v_mfma_f32_16x16x32_bf16) MLP
+// inference and training, vMF-mixture sampling entry points, deterministic
+// training-sample compaction and Adam.
+//
+// MFMA operand layout (16x16x32, one wave of 64 lanes; lane l):
+//   A (16 x 32):  row l%16, k = (l/16)*8 .. +7     (8 BF16 = 16 B per lane)
+//   B (32 x 16):  col l%16, k = (l/16)*8 .. +7
+//   D (16 x 16):  col l%16, rows (l/16)*4 .. +3    (4 FP32)
+// Activations live in LDS as BF16 rows of kPitch elements (64 + 8 padding so
+// that the 16 B fragment reads of 16 consecutive rows spread over the banks);
+// weights are read as B fragments straight from global memory (32 KiB per
+// direction, resident in L1/L2).
+#include "dev_guide.h"
+#include "dev_shade.h"
+#include "guide_kernels.h"
+#include "pt_kernels.h"
+
+namespace lfhd {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kPitch = 72;            // BF16 elements per activation row
+constexpr int kOutPitch = 41;         // FP32 elements per output row
+constexpr int kInferWaves = 2;        // waves per inference block
+constexpr int kInferBlock = 64 * kInferWaves;
+constexpr int kTrainRows = 64;        // samples per training block
+constexpr int kTrainBlock = 256;      // 4 waves: one 16-row M tile each
+constexpr int kPartStride = kGuideParams + 1;  // per-block partial: grads + loss
+
+LFH_DEV bf16x8 ld_frag(const uint16_t* p) { return *reinterpret_cast<const bf16x8*>(p); }
+
+LFH_DEV f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
+
+// Writes the 64 encoded features of one query as BF16 into an LDS row.
+LFH_DEV void encode_row(uint16_t* row, const GuideDev& g, V3 p, V3 n, V3 wo) {
+    const V3 x = v3(guide_norm_coord(p.x, g.lo[0], g.inv_ext[0]), guide_norm_coord(p.y, g.lo[1], g.inv_ext[1]),
+                    guide_norm_coord(p.z, g.lo[2], g.inv_ext[2]));
+#pragma unroll 4
+    for (int f = 0; f < kGuideIn; ++f) row[f] = to_bf16(guide_feature(f, x, n, wo));
+}
+
+// Forward pass of the 64 rows of X owned by this wave (rows r0 .. r0+63 in
+// 16-row tiles), in place: hidden layers overwrite X, the 40 outputs go to O
+// (FP32, kOutPitch).  Each 16-row tile reads only its own rows, so the
+// in-place update is safe within the wave.
+LFH_DEV void mlp_forward_rows(const GuideDev& g, uint16_t* X, float* O, int ntiles) {
+    const int lane = threadIdx.x & 63;
+    const int lr = lane & 15, lk = (lane >> 4) * 8, lq = (lane >> 4) * 4;
+#pragma unroll
+    for (int layer = 0; layer < kGuideLayers; ++layer) {
+        const uint16_t* W = g.wf + layer * 4096;
+        const float* bias = g.bias + layer * 64;
+        constexpr int kN[4] = {4, 4, 4, 3};
+        const int nN = kN[layer];
+        for (int mt = 0; mt < ntiles; ++mt) {
+            f32x4 acc[4];
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int ks = 0; ks < 2; ++ks) {
+                const bf16x8 a = ld_frag(X + (mt * 16 + lr) * kPitch + ks * 32 + lk);
+#pragma unroll
+                for (int nt = 0; nt < 4; ++nt)
+                    if (nt < nN) acc[nt] = mfma(a, ld_frag(W + (nt * 16 + lr) * 64 + ks * 32 + lk), acc[nt]);
+            }
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) {
+                if (nt >= nN) continue;
+                const int col = nt * 16 + lr;
+                const float b = bias[col];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int row = mt * 16 + lq + j;
+                    const float v = acc[nt][j] + b;
+                    if (layer < 3) X[row * kPitch + col] = to_bf16(fmaxf(v, 0.0f));
+                    else if (col < kGuideOut) O[row * kOutPitch + col] = v;
+                }
+            }
+        }
+    }
+}
+
+// ---- inference ------------------------------------------------------------------
+// Source of the queries: host-provided (position, normal, wo) triples, or the
+// current hit queue of the path tracer.
+struct InferQueue {
+    ShadeScene sc;
+    const float4* rayq;
+    const float4* hitq;
+    const uint32_t* count;
+    GuideHook gh;
+    int depth;
+};
+
+// Query inputs of hit-queue entry i: the hit point, the face-forwarded
+// geometric normal in world space, and wo = -dir.  Returns false for misses
+// and light hits (no lobes needed).
+LFH_DEV bool queue_query(const InferQueue& q, uint32_t i, V3& pos, V3& nrm, V3& wo, int& path) {
+    const float4 r0 = q.rayq[(size_t)i * 2], r1 = q.rayq[(size_t)i * 2 + 1];
+    const float4 h0 = q.hitq[(size_t)i * 2], h1 = q.hitq[(size_t)i * 2 + 1];
+    path = __float_as_int(r0.w);
+    const V3 ro = ld3(r0), rd = ld3(r1);
+    wo = -rd;
+    pos = ro + rd * h0.x;
+    const int slot = __float_as_int(h0.w);
+    if (h0.x == kInfinity || slot < 0) {
+        nrm = v3(0.f, 0.f, 1.f);
+        return false;
+    }
+    const int inst = __float_as_int(h1.x);
+    const float4* tri = q.sc.tr.tris + (size_t)slot * 3;
+    const V3 e0 = ld3(tri[1]), e1 = ld3(tri[2]);
+    const V3 no = cross(e0, e1);
+    const float4* np = q.sc.nmats + (size_t)inst * 3;
+    const float4 m0 = np[0], m1 = np[1], m2 = np[2];
+    V3 n = v3(m0.x * no.x + m0.y * no.y + m0.z * no.z, m1.x * no.x + m1.y * no.y + m1.z * no.z,
+              m2.x * no.x + m2.y * no.y + m2.z * no.z);
+    const float l2 = dot(n, n);
+    n = l2 > 0.0f ? n * (1.0f / sqrtf(l2)) : v3(0.f, 0.f, 1.f);
+    nrm = dot(n, rd) > 0.0f ? -n : n;
+    return true;
+}
+
+template <bool QUEUE>
+__global__ __launch_bounds__(kInferBlock) void k_guide_infer(GuideDev g, const float* __restrict__ in9, int n_host,
+                                                             InferQueue q, float* __restrict__ raw40,
+                                                             float4* __restrict__ lobes) {
+    __shared__ __attribute__((aligned(16))) uint16_t sX[kInferWaves][64 * kPitch];
+    __shared__ float sO[kInferWaves][64 * kOutPitch];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t n = QUEUE ? *q.count : (uint32_t)n_host;
+    if ((uint32_t)blockIdx.x * kInferBlock >= n) return;  // block-uniform
+    const uint32_t i = (uint32_t)blockIdx.x * kInferBlock + threadIdx.x;
+    const bool live = i < n;
+    V3 pos = v3(0.f, 0.f, 0.f), nrm = v3(0.f, 0.f, 1.f), wo = v3(0.f, 0.f, 1.f);
+    if (live) {
+        if (QUEUE) {
+            int path;
+            const bool surf = queue_query(q, i, pos, nrm, wo, path);
+            const GuideHook& gh = q.gh;
+            if (surf && gh.train && (path % gh.stride) == gh.phase) {
+                float4* r = gh.rec + ((size_t)(path / gh.stride) * gh.max_depth + q.depth) * 4;
+                r[0] = make_float4(pos.x, pos.y, pos.z, 0.0f);
+                r[1] = make_float4(nrm.x, nrm.y, nrm.z, 0.0f);
+                r[2] = make_float4(wo.x, wo.y, wo.z, -1.0f);
+            }
+        } else {
+            const float* s = in9 + (size_t)i * 9;
+            pos = v3(s[0], s[1], s[2]);
+            nrm = v3(s[3], s[4], s[5]);
+            wo = v3(s[6], s[7], s[8]);
+        }
+    }
+    uint16_t* X = sX[wave];
+    float* O = sO[wave];
+    encode_row(X + lane * kPitch, g, pos, nrm, wo);
+    __syncthreads();
+    mlp_forward_rows(g, X, O, 4);
+    __syncthreads();
+    if (live) {
+        const float* o = O + lane * kOutPitch;
+        if (raw40) {
+            float* dst = raw40 + (size_t)i * kGuideOut;
+            for (int k = 0; k < kGuideOut; ++k) dst[k] = o[k];
+        }
+        if (lobes) {
+            float ov[kGuideOut];
+#pragma unroll
+            for (int k = 0; k < kGuideOut; ++k) ov[k] = o[k];
+            float4 lb[kGuideLobeF4];
+            guide_lobes(ov, lb);
+            float4* dst = lobes + (size_t)i * kGuideLobeF4;
+#pragma unroll
+            for (int k = 0; k < kGuideLobeF4; ++k) dst[k] = lb[k];
+        }
+    }
+}
+
+// ---- mixture sampling / pdf on processed lobes (checks of the device sampler) ----
+__global__ void k_guide_sample(const float4* __restrict__ lobes, const uint4* __restrict__ seeds, int n,
+                               float* __restrict__ dirs, float* __restrict__ pdf) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint4 s = seeds[i];
+    const float u0 = rnd(s), u1 = rnd(s), u2 = rnd(s);
+    const float4* lb = lobes + (size_t)i * kGuideLobeF4;
+    const V3 d = guide_sample(lb, u0, u1, u2);
+    dirs[(size_t)i * 3] = d.x;
+    dirs[(size_t)i * 3 + 1] = d.y;
+    dirs[(size_t)i * 3 + 2] = d.z;
+    pdf[i] = guide_pdf(lb, d);
+}
+
+__global__ void k_guide_pdf(const float4* __restrict__ lobes, const float* __restrict__ dirs, int n,
+                            float* __restrict__ pdf) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    pdf[i] = guide_pdf(lobes + (size_t)i * kGuideLobeF4, v3(dirs[(size_t)i * 3], dirs[(size_t)i * 3 + 1],
+                                                              dirs[(size_t)i * 3 + 2]));
+}
+
+// ---- training -----------------------------------------------------------------------
+// One block = 64 samples (4 waves x one 16-row tile).  Forward with the
+// activations kept in LDS, analytic d loss / d outputs per sample, then
+// backward: weight gradients dW = X^T dY (K = the block's samples) and
+// dX = (dY W^T) masked by ReLU, all as MFMA tiles.  Each block writes its
+// partial gradient; k_guide_reduce_adam sums the partials in block order, so
+// the result does not depend on scheduling.
+//
+// Samples: 4 float4 each: (pos, w), (n, 0), (wo, 0), (d, 0).
+struct TrainArgs {
+    const float4* samples;
+    const int* count;      // device: number of valid samples
+    int start;             // first sample of this batch
+    int batch;             // batch capacity (blocks x 64)
+    float* partial;        // [blocks][kPartStride]
+};
+
+LFH_DEV int batch_size(const TrainArgs& t) {
+    int n = *t.count - t.start;
+    return n < 0 ? 0 : (n > t.batch ? t.batch : n);
+}
+
+__global__ __launch_bounds__(kTrainBlock) void k_guide_train(GuideDev g, TrainArgs ta) {
+    __shared__ __attribute__((aligned(16))) uint16_t sA[4][kTrainRows * kPitch];  // E, H1, H2, H3
+    __shared__ __attribute__((aligned(16))) uint16_t sG0[kTrainRows * kPitch];
+    // the FP32 outputs (forward, loss) and the second gradient buffer
+    // (backward) are never live together
+    __shared__ __attribute__((aligned(16))) float sOG[kTrainRows * kOutPitch];
+    static_assert(sizeof(float) * kTrainRows * kOutPitch >= sizeof(uint16_t) * kTrainRows * kPitch, "alias size");
+    float* sO = sOG;
+    uint16_t* sG1 = reinterpret_cast<uint16_t*>(sOG);
+    __shared__ float sD[kTrainRows][4];
+    __shared__ float sLoss[kTrainRows];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int lr = lane & 15, lk = (lane >> 4) * 8, lq = (lane >> 4) * 4;
+    const int N = batch_size(ta);
+    const int base = blockIdx.x * kTrainRows;
+    float* part = ta.partial + (size_t)blockIdx.x * kPartStride;
+    if (base >= N) return;  // block-uniform; the reduction reads only active blocks
+
+    // 1. encode: thread (s = tid & 63, quarter = tid >> 6) writes 16 features of sample s
+    {
+        const int s = tid & 63, qtr = tid >> 6;
+        const bool ok = base + s < N;
+        V3 p = v3(0.f, 0.f, 0.f), nn = v3(0.f, 0.f, 1.f), wo = v3(0.f, 0.f, 1.f);
+        float w = 0.0f;
+        if (ok) {
+            const float4* sp = ta.samples + (size_t)(ta.start + base + s) * 4;
+            const float4 a = sp[0], b = sp[1], c = sp[2], d = sp[3];
+            p = ld3(a);
+            w = a.w;
+            nn = ld3(b);
+            wo = ld3(c);
+            if (qtr == 0) {
+                sD[s][0] = d.x;
+                sD[s][1] = d.y;
+                sD[s][2] = d.z;
+            }
+        }
+        if (qtr == 0) sD[s][3] = ok ? w : 0.0f;
+        const V3 x = v3(guide_norm_coord(p.x, g.lo[0], g.inv_ext[0]), guide_norm_coord(p.y, g.lo[1], g.inv_ext[1]),
+                        guide_norm_coord(p.z, g.lo[2], g.inv_ext[2]));
+        uint16_t* row = sA[0] + s * kPitch;
+        for (int f = qtr * 16; f < qtr * 16 + 16; ++f) row[f] = ok ? to_bf16(guide_feature(f, x, nn, wo)) : (uint16_t)0;
+    }
+    __syncthreads();
+
+    // 2. forward, wave w on rows 16w..16w+15; layer l reads sA[l], writes sA[l+1] (or sO)
+#pragma unroll
+    for (int layer = 0; layer < kGuideLayers; ++layer) {
+        const uint16_t* W = g.wf + layer * 4096;
+        const float* bias = g.bias + layer * 64;
+        const int nN = layer == 3 ? 3 : 4;
+        const uint16_t* X = sA[layer];
+        f32x4 acc[4];
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+            const bf16x8 a = ld_frag(X + (wave * 16 + lr) * kPitch + ks * 32 + lk);
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt)
+                if (nt < nN) acc[nt] = mfma(a, ld_frag(W + (nt * 16 + lr) * 64 + ks * 32 + lk), acc[nt]);
+        }
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) {
+            if (nt >= nN) continue;
+            const int col = nt * 16 + lr;
+            const float b = bias[col];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int row = wave * 16 + lq + j;
+                const float v = acc[nt][j] + b;
+                if (layer < 3) sA[layer + 1][row * kPitch + col] = to_bf16(fmaxf(v, 0.0f));
+                else if (col < kGuideOut) sO[row * kOutPitch + col] = v;
+            }
+        }
+    }
+    __syncthreads();
+
+    // 3. d loss / d outputs (one thread per sample), BF16 into sG0 (cols 40..63 zero)
+    const float invN = 1.0f / (float)N;
+    if (tid < kTrainRows) {
+        const int s = tid;
+        const float* o = sO + s * kOutPitch;
+        uint16_t* grow = sG0 + s * kPitch;
+        const float w = sD[s][3];
+        float lossv = 0.0f;
+        for (int c = kGuideOut; c < 64; ++c) grow[c] = 0;
+        if (w != 0.0f) {
+            const V3 d = v3(sD[s][0], sD[s][1], sD[s][2]);
+            float amax = -3.0e38f;
+            for (int k = 0; k < kGuideLobes; ++k) amax = fmaxf(amax, o[5 * k + 4]);
+            float pi[kGuideLobes], lw[kGuideLobes], kap[kGuideLobes], cs[kGuideLobes], rn[kGuideLobes];
+            V3 mu[kGuideLobes];
+            float psum = 0.0f;
+            for (int k = 0; k < kGuideLobes; ++k) {
+                pi[k] = expf(o[5 * k + 4] - amax);
+                psum += pi[k];
+            }
+            float lmax = -3.0e38f;
+            for (int k = 0; k < kGuideLobes; ++k) {
+                pi[k] /= psum;
+                const V3 r = v3(o[5 * k], o[5 * k + 1], o[5 * k + 2]);
+                rn[k] = fmaxf(sqrtf(dot(r, r)), 1e-12f);
+                mu[k] = r * (1.0f / rn[k]);
+                kap[k] = expf(fminf(fmaxf(o[5 * k + 3], kLogKMin), kLogKMax));
+                cs[k] = dot(mu[k], d);
+                lw[k] = logf(pi[k]) + logf(vmf_norm(kap[k])) + kap[k] * (cs[k] - 1.0f);
+                lmax = fmaxf(lmax, lw[k]);
+            }
+            float gs = 0.0f;
+            for (int k = 0; k < kGuideLobes; ++k) gs += expf(lw[k] - lmax);
+            lossv = -w * (lmax + logf(gs)) * invN;
+            const float sc = w * invN;
+            for (int k = 0; k < kGuideLobes; ++k) {
+                const float gam = expf(lw[k] - lmax) / gs;
+                const float e2 = expf(-2.0f * kap[k]);
+                const float dlogv = 1.0f / kap[k] - 2.0f * e2 / (-expm1f(-2.0f * kap[k])) + (cs[k] - 1.0f);
+                const float sraw = o[5 * k + 3];
+                const bool inside = sraw > kLogKMin && sraw < kLogKMax;
+                const float gr = -sc * gam * kap[k] / rn[k];
+                grow[5 * k + 0] = to_bf16(gr * (d.x - cs[k] * mu[k].x));
+                grow[5 * k + 1] = to_bf16(gr * (d.y - cs[k] * mu[k].y));
+                grow[5 * k + 2] = to_bf16(gr * (d.z - cs[k] * mu[k].z));
+                grow[5 * k + 3] = to_bf16(inside ? -sc * gam * kap[k] * dlogv : 0.0f);
+                grow[5 * k + 4] = to_bf16(sc * (pi[k] - gam));
+            }
+        } else {
+            for (int c = 0; c < kGuideOut; ++c) grow[c] = 0;
+        }
+        sLoss[s] = lossv;
+    }
+    __syncthreads();
+    if (tid < 64) {
+        float l = sLoss[tid];
+        for (int off = 32; off > 0; off >>= 1) l += __shfl_xor(l, off);
+        if (tid == 0) part[kGuideParams] = l;
+    }
+
+    // 4. backward, layers 3..0.  dY of layer l is in (l odd ? sG0 : sG1) — l = 3: sG0.
+#pragma unroll
+    for (int layer = 3; layer >= 0; --layer) {
+        uint16_t* dY = (layer & 1) ? sG0 : sG1;
+        uint16_t* dX = (layer & 1) ? sG1 : sG0;
+        const uint16_t* X = sA[layer];
+        const int nout = guide_layer_out(layer);
+        const int off = guide_layer_offset(layer);
+        // dW[i][o] = sum_s X[s][i] dY[s][o]: wave w computes in-tile it = w for every out tile
+        {
+            const int it = wave;
+            const int nOt = layer == 3 ? 3 : 4;
+#pragma unroll
+            for (int ot = 0; ot < 4; ++ot) {
+                if (ot >= nOt) continue;
+                f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int ks = 0; ks < 2; ++ks) {
+                    bf16x8 a, b;
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        const int s = ks * 32 + lk + j;
+                        uint16_t xa = X[s * kPitch + it * 16 + lr];
+                        uint16_t yb = dY[s * kPitch + ot * 16 + lr];
+                        a[j] = __builtin_bit_cast(__bf16, xa);
+                        b[j] = __builtin_bit_cast(__bf16, yb);
+                    }
+                    acc = mfma(a, b, acc);
+                }
+                const int col = ot * 16 + lr;
+                if (col < nout) {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const int in = it * 16 + lq + j;
+                        part[off + in * nout + col] = acc[j];
+                    }
+                }
+            }
+        }
+        // db[o] = sum_s dY[s][o]
+        if (tid < nout) {
+            float sum = 0.0f;
+            for (int s = 0; s < kTrainRows; ++s) sum += from_bf16(dY[s * kPitch + tid]);
+            part[off + 64 * nout + tid] = sum;
+        }
+        if (layer > 0) {
+            // dX[s][i] = (sum_o dY[s][o] W[i][o]) * (X[s][i] > 0), rows of this wave
+            const uint16_t* WB = g.wb + layer * 4096;
+            f32x4 acc[4];
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int ks = 0; ks < 2; ++ks) {
+                const bf16x8 a = ld_frag(dY + (wave * 16 + lr) * kPitch + ks * 32 + lk);
+#pragma unroll
+                for (int nt = 0; nt < 4; ++nt) acc[nt] = mfma(a, ld_frag(WB + (nt * 16 + lr) * 64 + ks * 32 + lk), acc[nt]);
+            }
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) {
+                const int col = nt * 16 + lr;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int row = wave * 16 + lq + j;
+                    const bool on = from_bf16(X[row * kPitch + col]) > 0.0f;
+                    dX[row * kPitch + col] = to_bf16(on ? acc[nt][j] : 0.0f);
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Sums the partials of the active blocks in block order, then either stores
+// the gradient (grad_out) or applies one Adam step (Kingma & Ba 2015,
+// Algorithm 1) to the FP32 master weights and refreshes the BF16 copies.
+struct AdamArgs {
+    float* master;
+    float* m;
+    float* v;
+    uint16_t* wf;
+    uint16_t* wb;
+    float* bias;
+    float lr, b1, b2, eps;
+    float c1, c2;   // 1 / (1 - b1^t), 1 / (1 - b2^t)
+};
+
+__global__ void k_guide_reduce_adam(TrainArgs ta, AdamArgs ad, float* __restrict__ grad_out, float* __restrict__ loss_out) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j > kGuideParams) return;
+    const int N = batch_size(ta);
+    const int nb = (N + kTrainRows - 1) / kTrainRows;
+    float sum = 0.0f;
+    for (int b = 0; b < nb; ++b) sum += ta.partial[(size_t)b * kPartStride + j];
+    if (j == kGuideParams) {
+        if (loss_out) *loss_out = sum;
+        return;
+    }
+    if (grad_out) {
+        grad_out[j] = sum;
+        return;
+    }
+    if (nb == 0) return;
+    const float gm = ad.b1 * ad.m[j] + (1.0f - ad.b1) * sum;
+    const float gv = ad.b2 * ad.v[j] + (1.0f - ad.b2) * sum * sum;
+    ad.m[j] = gm;
+    ad.v[j] = gv;
+    const float w = ad.master[j] - ad.lr * (gm * ad.c1) / (sqrtf(gv * ad.c2) + ad.eps);
+    ad.master[j] = w;
+    guide_store_param(ad.wf, ad.wb, ad.bias, j, w);
+}
+
+__global__ void k_guide_refresh(const float* __restrict__ master, uint16_t* wf, uint16_t* wb, float* bias) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < kGuideParams) guide_store_param(wf, wb, bias, j, master[j]);
+}
+
+// ---- training-sample collection ----------------------------------------------------
+// Records (4 float4 per recorded path vertex): (pos, valid), (n, lum T_before),
+// (wo, lum R_base), (wi, 0).  A sample is emitted when the vertex continued
+// the path (valid) and the next vertex stored the radiance reached before it
+// (R_base >= 0); its weight is the luminance of the radiance added through wi,
+// relative to the throughput before the vertex: (lum R_final - R_base) / lum T.
+struct CollectArgs {
+    const float4* rec;
+    int nslots;        // recorded paths x max_depth
+    int max_depth;
+    int stride, phase;
+    const float4* rad; // final path radiance
+    int npaths;
+    int* block_counts;
+    int* block_offsets;
+    int* total;
+    float4* samples;
+    int max_samples;
+};
+
+LFH_DEV bool collect_one(const CollectArgs& c, int slot, float4* out) {
+    if (slot >= c.nslots) return false;
+    const float4* r = c.rec + (size_t)slot * 4;
+    const float4 r0 = r[0];
+    if (r0.w != 1.0f) return false;
+    const float4 r1 = r[1], r2 = r[2];
+    if (r2.w < 0.0f || !(r1.w > 0.0f)) return false;
+    const int path = (slot / c.max_depth) * c.stride + c.phase;
+    if (path >= c.npaths) return false;
+    const float4 f = c.rad[path];
+    const float lf = 0.2126f * f.x + 0.7152f * f.y + 0.0722f * f.z;
+    const float w = (lf - r2.w) / r1.w;
+    if (!(w > 0.0f) || !isfinite(w)) return false;
+    if (out) {
+        const float4 r3 = r[3];
+        out[0] = make_float4(r0.x, r0.y, r0.z, w);
+        out[1] = make_float4(r1.x, r1.y, r1.z, 0.0f);
+        out[2] = make_float4(r2.x, r2.y, r2.z, 0.0f);
+        out[3] = make_float4(r3.x, r3.y, r3.z, 0.0f);
+    }
+    return true;
+}
+
+__global__ __launch_bounds__(256) void k_guide_count(CollectArgs c) {
+    const int slot = blockIdx.x * 256 + threadIdx.x;
+    const int ok = collect_one(c, slot, nullptr) ? 1 : 0;
+    const int cnt = __syncthreads_count(ok);
+    if (threadIdx.x == 0) c.block_counts[blockIdx.x] = cnt;
+}
+
+// Exclusive scan of the block counts (one block of 1024 threads).
+__global__ __launch_bounds__(1024) void k_guide_scan(CollectArgs c, int nblocks) {
+    __shared__ int s[1024];
+    const int t = threadIdx.x;
+    const int per = (nblocks + 1023) / 1024;
+    const int b0 = t * per;
+    int local = 0;
+    for (int b = b0; b < b0 + per && b < nblocks; ++b) local += c.block_counts[b];
+    s[t] = local;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int v = t >= off ? s[t - off] : 0;
+        __syncthreads();
+        s[t] += v;
+        __syncthreads();
+    }
+    int run = s[t] - local;
+    for (int b = b0; b < b0 + per && b < nblocks; ++b) {
+        c.block_offsets[b] = run;
+        run += c.block_counts[b];
+    }
+    if (t == 1023) *c.total = s[1023] < c.max_samples ? s[1023] : c.max_samples;
+}
+
+__global__ __launch_bounds__(256) void k_guide_compact(CollectArgs c) {
+    __shared__ int sw[4];
+    const int slot = blockIdx.x * 256 + threadIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const bool ok = collect_one(c, slot, nullptr);
+    const uint64_t m = __ballot(ok);
+    if (lane == 0) sw[wave] = __popcll(m);
+    __syncthreads();
+    int off = c.block_offsets[blockIdx.x];
+    for (int w = 0; w < wave; ++w) off += sw[w];
+    off += __popcll(m & ((1ull << lane) - 1ull));
+    if (ok && off < c.max_samples) collect_one(c, slot, c.samples + (size_t)off * 4);
+}
+
+// ---- launchers ------------------------------------------------------------------------
+static inline int nblk(int64_t n, int b) { return (int)((n + b - 1) / b); }
+
+hipError_t guide_forward_host(const GuideDev& g, const float* in9, int n, float* raw40, float4* lobes,
+                              hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    InferQueue q{};
+    hipLaunchKernelGGL(k_guide_infer<false>, dim3(nblk(n, kInferBlock)), dim3(kInferBlock), 0, s, g, in9, n, q, raw40,
+                       lobes);
+    return hipGetLastError();
+}
+
+hipError_t guide_infer_queue(const GuideDev& g, const ShadeScene& sc, const float4* rayq, const float4* hitq,
+                             const uint32_t* count, int maxq, const GuideHook& gh, int depth, float4* lobes,
+                             hipStream_t s) {
+    if (maxq <= 0) return hipSuccess;
+    InferQueue q{sc, rayq, hitq, count, gh, depth};
+    hipLaunchKernelGGL(k_guide_infer<true>, dim3(nblk(maxq, kInferBlock)), dim3(kInferBlock), 0, s, g,
+                       (const float*)nullptr, 0, q, (float*)nullptr, lobes);
+    return hipGetLastError();
+}
+
+hipError_t guide_sample(const float4* lobes, const uint4* seeds, int n, float* dirs, float* pdf, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_guide_sample, dim3(nblk(n, 256)), dim3(256), 0, s, lobes, seeds, n, dirs, pdf);
+    return hipGetLastError();
+}
+
+hipError_t guide_pdf(const float4* lobes, const float* dirs, int n, float* pdf, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_guide_pdf, dim3(nblk(n, 256)), dim3(256), 0, s, lobes, dirs, n, pdf);
+    return hipGetLastError();
+}
+
+int guide_train_blocks(int batch) { return nblk(batch, kTrainRows); }
+size_t guide_partial_floats(int batch) { return (size_t)guide_train_blocks(batch) * kPartStride; }
+
+hipError_t guide_train(const GuideDev& g, const float4* samples, const int* count, int start, int batch,
+                       float* partial, const GuideAdam& adam, float* grad_out, float* loss_out, hipStream_t s) {
+    TrainArgs ta{samples, count, start, batch, partial};
+    hipLaunchKernelGGL(k_guide_train, dim3(guide_train_blocks(batch)), dim3(kTrainBlock), 0, s, g, ta);
+    AdamArgs ad{adam.master, adam.m, adam.v, adam.wf, adam.wb, adam.bias, adam.lr, adam.b1, adam.b2, adam.eps,
+                adam.c1, adam.c2};
+    hipLaunchKernelGGL(k_guide_reduce_adam, dim3(nblk(kGuideParams + 1, 256)), dim3(256), 0, s, ta, ad, grad_out,
+                       loss_out);
+    return hipGetLastError();
+}
+
+hipError_t guide_refresh(const float* master, uint16_t* wf, uint16_t* wb, float* bias, hipStream_t s) {
+    hipLaunchKernelGGL(k_guide_refresh, dim3(nblk(kGuideParams, 256)), dim3(256), 0, s, master, wf, wb, bias);
+    return hipGetLastError();
+}
+
+hipError_t guide_collect(const float4* rec, int nslots, int max_depth, int stride, int phase, const float4* rad,
+                         int npaths, int* block_counts, int* block_offsets, int* total, float4* samples,
+                         int max_samples, hipStream_t s) {
+    CollectArgs c{rec, nslots, max_depth, stride, phase, rad, npaths, block_counts, block_offsets, total, samples,
+                  max_samples};
+    const int nb = nblk(nslots, 256);
+    if (nb <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_guide_count, dim3(nb), dim3(256), 0, s, c);
+    hipLaunchKernelGGL(k_guide_scan, dim3(1), dim3(1024), 0, s, c, nb);
+    hipLaunchKernelGGL(k_guide_compact, dim3(nb), dim3(256), 0, s, c);
+    return hipGetLastError();
+}
+
+int guide_collect_blocks(int nslots) { return nblk(nslots, 256); }
+
+}  // namespace lfhd

@@ -14,18 +14,18 @@
 // path state and is consumed in the reference's order; next-event terms are
 // evaluated before their shadow test (they consume no random numbers) and
 // added after it, in the reference's summation order.
+#include "dev_guide.h"
 #include "dev_shade.h"
+#include "guide_kernels.h"
 #include "pt_kernels.h"
 
 namespace lfhd {
 
-__device__ __forceinline__ void guide_record(const GuideHook& g, int p, int depth, V3 x, V3 n, V3 wo,
-                                             V3 wi, float pdf, V3 thr, bool alive) {
-    float4* r = g.rec + ((size_t)p * g.max_depth + depth) * 4;
-    r[0] = make_float4(x.x, x.y, x.z, alive ? 1.0f : 0.0f);
-    r[1] = make_float4(n.x, n.y, n.z, pdf);
-    r[2] = make_float4(wo.x, wo.y, wo.z, 0.0f);
-    r[3] = make_float4(wi.x, wi.y, wi.z, (thr.x + thr.y + thr.z) * (1.0f / 3.0f));
+LFH_DEV float luminance(V3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
+
+// Training record of path p at depth d (only for recorded paths).
+LFH_DEV float* guide_rec(const GuideHook& g, int p, int depth) {
+    return reinterpret_cast<float*>(g.rec + ((size_t)(p / g.stride) * g.max_depth + depth) * 4);
 }
 
 // ---- queue helpers ---------------------------------------------------------------
@@ -301,6 +301,8 @@ __global__ __launch_bounds__(256) void k_shade(ShadeScene sc1, const JobDev* __r
         V3 rad = ld3(pb.rad[p]);
         V3 thr = ld3(pb.thr[p]);
         combine_nee(pb, p, rad);
+        const bool recorded = gh.train && (p % gh.stride) == gh.phase;
+        if (recorded && depth > 0) guide_rec(gh, p, depth - 1)[11] = luminance(rad);
         float4 misc = pb.misc[p];
         float prevPdf = misc.x;
         int lastMat = __float_as_int(misc.y);
@@ -365,6 +367,8 @@ __global__ __launch_bounds__(256) void k_shade(ShadeScene sc1, const JobDev* __r
             lastMat = matId;
             thr = thr * v3(expf(-absorb.x * t), expf(-absorb.y * t), expf(-absorb.z * t));
             uint4 rng = pb.rng[p];
+            const bool guided = gh.enabled != 0;
+            const float4* lob = gh.lobes + (size_t)i * kGuideLobeF4;
             // next-event estimation (pathtrace.glsl:126-204)
             const V3 V = -rd;
             V3 surfacePos = fhp + nrm * kEps;
@@ -377,7 +381,8 @@ __global__ __launch_bounds__(256) void k_shade(ShadeScene sc1, const JobDev* __r
                 float bp;
                 V3 f = DisneyEval(m, fr.eta, V, fr.N, ldir, bp);
                 if (bp > 0.0f) {
-                    float mw = powerHeuristic(lightPdf, bp);
+                    const float qp = guided ? (1.0f - kGuideFrac) * bp + kGuideFrac * guide_pdf(lob, ldir) : bp;
+                    float mw = powerHeuristic(lightPdf, qp);
                     if (mw > 0.0f) {
                         V3 c = mw * f * fabsf(dot(ldir, fr.N)) * color / lightPdf;
                         e4 = f4(c, 1.0f);
@@ -396,7 +401,9 @@ __global__ __launch_bounds__(256) void k_shade(ShadeScene sc1, const JobDev* __r
                     V3 f = DisneyEval(m, fr.eta, V, fr.N, lr.direction, bp);
                     float area = sc.lights[(size_t)index * kLightF4 + 1].w;
                     float weight = 1.0f;
-                    if (area > 0.0f) weight = powerHeuristic(lr.pdf, bp);
+                    if (area > 0.0f)
+                        weight = powerHeuristic(
+                            lr.pdf, guided ? (1.0f - kGuideFrac) * bp + kGuideFrac * guide_pdf(lob, lr.direction) : bp);
                     if (bp > 0.0f) {
                         V3 c = weight * f * fabsf(dot(fr.N, lr.direction)) * lr.emission / lr.pdf;
                         l4 = f4(c, 1.0f);
@@ -413,28 +420,61 @@ __global__ __launch_bounds__(256) void k_shade(ShadeScene sc1, const JobDev* __r
             }
             // BSDF sampling (pathtrace.glsl:268-291)
             V3 L;
-            float pdf;
-            V3 f = DisneySample(m, fr, V, L, pdf, rng);
+            float pdf, misPdf;
+            V3 f;
+            if (guided) {
+                // one-sample MIS of the guide and the BSDF: the direction comes
+                // from either, weighted by the mixture density q = (1-a) p_bsdf + a p_guide
+                bool ok = true;
+                if (rnd(rng) < kGuideFrac) {
+                    const float u0 = rnd(rng), u1 = rnd(rng), u2 = rnd(rng);
+                    L = guide_sample(lob, u0, u1, u2);
+                } else {
+                    float pl;
+                    DisneySample(m, fr, V, L, pl, rng);
+                    ok = pl > 0.0f;
+                }
+                float bp;
+                f = DisneyEval(m, fr.eta, V, fr.N, L, bp);
+                pdf = ok ? (1.0f - kGuideFrac) * bp + kGuideFrac * guide_pdf(lob, L) : 0.0f;
+                misPdf = pdf;
+            } else {
+                f = DisneySample(m, fr, V, L, pdf, rng);
+                misPdf = pdf;
+                if (sc.estimator == 1 && pdf > 0.0f) {
+                    // consistent MIS: the BSDF strategy's density is the lobe
+                    // mixture's (disney.glsl:289-290), as in the light-sample weights
+                    float bp;
+                    DisneyEval(m, fr.eta, V, fr.N, L, bp);
+                    misPdf = bp;
+                }
+            }
             if (dot(fr.N, L) < 0.0f)
                 absorb = v3(-logf(m.extinction.x) / m.atDistance, -logf(m.extinction.y) / m.atDistance,
                             -logf(m.extinction.z) / m.atDistance);
             bool alive = pdf > 0.0f;
+            const V3 thrBefore = thr;
             if (alive) thr = thr * ((f * fabsf(dot(fr.N, L))) / pdf);
             if (alive && sc.rr && depth >= sc.rr_depth) {
                 float q = fminf(fmaxf(thr.x, fmaxf(thr.y, thr.z)) + 0.001f, 0.95f);
                 if (rnd(rng) > q) alive = false;
                 else thr = thr / q;
             }
-            if (gh.enabled) guide_record(gh, p, depth, fhp, nrm, V, L, pdf, thr, alive);
             if (alive && depth + 1 < sc.max_depth) {
                 cont = true;
                 nextD = L;
                 nextO = fhp + L * kEps;
             }
+            if (recorded) {
+                float* r = guide_rec(gh, p, depth);
+                r[3] = cont ? 1.0f : 0.0f;
+                r[7] = luminance(thrBefore);
+                reinterpret_cast<float4*>(r)[3] = make_float4(L.x, L.y, L.z, 0.0f);
+            }
             pb.rng[p] = rng;
             pb.thr[p] = f4(thr, 0.f);
             pb.absorb[p] = f4(absorb, 0.f);
-            prevPdf = pdf;
+            prevPdf = misPdf;
         }
         pb.rad[p] = f4(rad, 0.f);
         pb.misc[p] = make_float4(prevPdf, __int_as_float(lastMat), 0.f, 0.f);
@@ -480,6 +520,7 @@ __global__ __launch_bounds__(256) void k_accumulate(PathBufs pb, int total, CamP
     if (gx >= cam.width || gy >= cam.height) return;
     V3 rad = ld3(pb.rad[p]);
     combine_nee(pb, p, rad);
+    pb.rad[p] = f4(rad, 0.f);   // final path radiance (guide training targets)
     const size_t pix = (size_t)gy * cam.width + gx;
     float4 s = accum[pix];
     if (kahan) {
@@ -646,10 +687,15 @@ hipError_t pt_generate_rays(const float* rays6, const uint4* seeds, int n, const
 hipError_t pt_bounce(const ShadeScene& sc, int depth, int maxq, float4* rayq, uint32_t* count, float4* nextq,
                      uint32_t* next_count, float4* hitq, float4* shadowq, uint32_t* shadow_count, const PathBufs& pb,
                      uint32_t* spill, unsigned long long* work, int* err, float cull, int grid_blocks,
-                     const GuideHook& gh, hipStream_t s) {
+                     const GuideHook& gh, const lfhd::GuideDev* gd, hipStream_t s) {
     const JobDev* nj = nullptr;
     hipLaunchKernelGGL((k_trace_queue<false, false>), dim3(grid_blocks), dim3(kTraceBlock), 0, s, sc, nj, rayq, count,
                        hitq, pb, spill, work, cull, err);
+    if (gh.enabled) {
+        hipError_t e = guide_infer_queue(*gd, sc, rayq, hitq, count, maxq, gh, depth,
+                                         const_cast<float4*>(gh.lobes), s);
+        if (e != hipSuccess) return e;
+    }
     hipLaunchKernelGGL(k_shade<false>, dim3(nblk(maxq)), dim3(256), 0, s, sc, nj, depth, rayq, hitq, count, nextq,
                        next_count, shadowq, shadow_count, pb, gh);
     hipLaunchKernelGGL((k_trace_queue<true, false>), dim3(grid_blocks), dim3(kTraceBlock), 0, s, sc, nj, shadowq,
@@ -663,7 +709,7 @@ hipError_t pt_bounce_multi(const JobDev* jobs, int depth, int maxq, float4* rayq
                            const PathBufs& pb, uint32_t* spill, unsigned long long* work, int* err, float cull,
                            int grid_blocks, hipStream_t s) {
     ShadeScene dummy{};
-    GuideHook gh{0, 0, nullptr};
+    GuideHook gh{};
     hipLaunchKernelGGL((k_trace_queue<false, true>), dim3(grid_blocks), dim3(kTraceBlock), 0, s, dummy, jobs, rayq,
                        count, hitq, pb, spill, work, cull, err);
     hipLaunchKernelGGL(k_shade<true>, dim3(nblk(maxq)), dim3(256), 0, s, dummy, jobs, depth, rayq, hitq, count, nextq,

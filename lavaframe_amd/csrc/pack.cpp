@@ -393,6 +393,7 @@ lfhd::ShadeScene shade_view(const PackedScene& ps, const DeviceScene& ds, const 
     s.bg[0] = o.bg_color[0];
     s.bg[1] = o.bg_color[1];
     s.bg[2] = o.bg_color[2];
+    s.estimator = o.estimator;
     return s;
 }
 

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "dev_guide.h"
 #include "pt_types.h"
 
 // Path state, indexed by path id (tile pixel).
@@ -16,14 +17,25 @@ struct PathBufs {
     int* job;        // job id per path (multi-job passes only)
 };
 
-// Training-sample hook of the learned guide (disabled when enabled == 0):
-// records per path vertex (position, normal, wo, sampled wi, BSDF pdf) and the
-// throughput, for the radiance targets assembled after the pass.
+// Learned-guide hook of a pass.
+//   enabled  the shade stage draws from the guide (lobes of the current hit
+//            queue, kGuideLobeF4 float4 per entry) mixed with the BSDF
+//   train    training records are written for paths p with p % stride == phase:
+//            4 float4 per (recorded path, depth) = (pos, continued),
+//            (n, lum throughput before the vertex), (wo, lum radiance before
+//            the next vertex), (wi, 0)
+namespace lfhd {
 struct GuideHook {
-    int enabled;
-    int max_depth;
-    float4* rec;     // 4 float4 per (path, depth)
+    int enabled = 0;
+    int train = 0;
+    int max_depth = 0;
+    int stride = 1;
+    int phase = 0;
+    float4* rec = nullptr;
+    const float4* lobes = nullptr;
 };
+}  // namespace lfhd
+using lfhd::GuideHook;
 
 
 hipError_t pt_reset(uint32_t* counters, unsigned long long* work, hipStream_t s);
@@ -33,7 +45,7 @@ hipError_t pt_generate_rays(const float* rays6, const uint4* seeds, int n, const
 hipError_t pt_bounce(const lfhd::ShadeScene& sc, int depth, int maxq, float4* rayq, uint32_t* count, float4* nextq,
                      uint32_t* next_count, float4* hitq, float4* shadowq, uint32_t* shadow_count, const PathBufs& pb,
                      uint32_t* spill, unsigned long long* work, int* err, float cull, int grid_blocks,
-                     const GuideHook& gh, hipStream_t s);
+                     const GuideHook& gh, const lfhd::GuideDev* gd, hipStream_t s);
 // Multi-job pass (serving): every job renders its full frame once; rays of all
 // jobs share the queues.  jobs: device table of njobs entries.
 hipError_t pt_generate_multi(const lfhd::JobDev* jobs, int njobs, int total, const PathBufs& pb, float4* rayq,

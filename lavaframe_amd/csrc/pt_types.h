@@ -33,6 +33,7 @@ struct ShadeScene {
     int use_env;              // ENVMAP && !CONSTANT_BG
     int const_bg;             // CONSTANT_BG
     float bg[3];
+    int estimator;            // 0: reference MIS weights; 1: mixture pdf after BSDF samples
 };
 
 // Camera uniforms + tile / frame of one pass (renderer.glsl inputs).

@@ -64,7 +64,7 @@ class Options(C.Structure):
         ("max_depth", C.c_int32), ("enable_rr", C.c_int32), ("rr_depth", C.c_int32),
         ("use_env_map", C.c_int32), ("use_constant_bg", C.c_int32), ("bg_color", C.c_float * 3),
         ("hdr_multiplier", C.c_float), ("kahan", C.c_int32), ("guiding", C.c_int32),
-        ("reserved", C.c_int32 * 6),
+        ("estimator", C.c_int32), ("reserved", C.c_int32 * 5),
     ]
 
 
@@ -148,6 +148,10 @@ _SIGS = {
     "lfh_guide_loss_grad": (C.c_int, [_P, f32p, f32p, f32p, C.c_int64, f32p, f32p]),
     "lfh_guide_train_step": (C.c_int, [_P, f32p, f32p, f32p, C.c_int64, f32p]),
     "lfh_set_guide": (C.c_int, [_P, _P, C.c_int32]),
+    "lfh_guide_steps": (C.c_int, [_P, i64p]),
+    "lfh_guide_sample": (C.c_int, [_P, f32p, u32p, C.c_int64, f32p, f32p]),
+    "lfh_guide_pdf": (C.c_int, [_P, f32p, f32p, C.c_int64, f32p]),
+    "lfh_guide_samples_seen": (C.c_int, [_P, i64p]),
 }
 
 _lib = None
@@ -552,6 +556,15 @@ class Context:
         self._chk(self.lib.lfh_env_pdf(self.h, fptr(d), d.shape[0], fptr(pdf)))
         return pdf
 
+    def set_guide(self, guide: "Guide | None", train_every: int = 1):
+        self._guide = guide  # keep the guide alive while attached
+        self._chk(self.lib.lfh_set_guide(self.h, guide.h if guide else None, int(train_every)))
+
+    def guide_samples_seen(self):
+        n = C.c_int64()
+        self._chk(self.lib.lfh_guide_samples_seen(self.h, C.byref(n)))
+        return n.value
+
     def path_trace_rays(self, rays6, seeds):
         r = np.ascontiguousarray(rays6, np.float32)
         sd = np.ascontiguousarray(seeds, np.uint32).copy()
@@ -559,6 +572,94 @@ class Context:
         rad = np.zeros((n, 3), np.float32)
         self._chk(self.lib.lfh_path_trace_rays(self.h, fptr(r), uptr(sd), n, fptr(rad)))
         return rad
+
+
+class Guide:
+    """Learned path guide (lfh_guide_*): MLP over a 64-feature encoding of
+    (position, normal, wo) -> 8 vMF lobes; BF16 MFMA kernels, Adam."""
+
+    N_PARAMS = 15080
+
+    def __init__(self, scene_min, scene_max, seed: int = 0, lr: float = 1e-3, lib=None):
+        self.lib = lib or load()
+        h = C.c_void_p()
+        lo = np.ascontiguousarray(scene_min, np.float32)
+        hi = np.ascontiguousarray(scene_max, np.float32)
+        check(self.lib.lfh_guide_create(fptr(lo), fptr(hi), int(seed), float(lr), C.byref(h)), None, self.lib)
+        self.h = h
+
+    @classmethod
+    def for_scene(cls, scene: "Scene", **kw):
+        v = scene.arrays()["vertices"][:, :3]
+        return cls(v.min(0), v.max(0), **kw)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.lfh_guide_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, code):
+        return check(code, None, self.lib)
+
+    def params(self):
+        p = np.zeros(self.N_PARAMS, np.float32)
+        n = C.c_int64(self.N_PARAMS)
+        self._chk(self.lib.lfh_guide_get_params(self.h, fptr(p), C.byref(n)))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        self._chk(self.lib.lfh_guide_set_params(self.h, fptr(p), p.size))
+
+    def forward(self, inputs9):
+        x = np.ascontiguousarray(inputs9, np.float32)
+        out = np.zeros((x.shape[0], 40), np.float32)
+        self._chk(self.lib.lfh_guide_forward(self.h, fptr(x), x.shape[0], fptr(out)))
+        return out
+
+    def sample(self, inputs9, seeds):
+        x = np.ascontiguousarray(inputs9, np.float32)
+        sd = np.ascontiguousarray(seeds, np.uint32)
+        n = x.shape[0]
+        d = np.zeros((n, 3), np.float32)
+        pdf = np.zeros(n, np.float32)
+        self._chk(self.lib.lfh_guide_sample(self.h, fptr(x), uptr(sd), n, fptr(d), fptr(pdf)))
+        return d, pdf
+
+    def pdf(self, inputs9, dirs):
+        x = np.ascontiguousarray(inputs9, np.float32)
+        d = np.ascontiguousarray(dirs, np.float32)
+        pdf = np.zeros(x.shape[0], np.float32)
+        self._chk(self.lib.lfh_guide_pdf(self.h, fptr(x), fptr(d), x.shape[0], fptr(pdf)))
+        return pdf
+
+    def loss_grad(self, inputs9, dirs, weights):
+        x = np.ascontiguousarray(inputs9, np.float32)
+        d = np.ascontiguousarray(dirs, np.float32)
+        w = np.ascontiguousarray(weights, np.float32)
+        loss = np.zeros(1, np.float32)
+        g = np.zeros(self.N_PARAMS, np.float32)
+        self._chk(self.lib.lfh_guide_loss_grad(self.h, fptr(x), fptr(d), fptr(w), x.shape[0], fptr(loss), fptr(g)))
+        return float(loss[0]), g
+
+    def train_step(self, inputs9, dirs, weights):
+        x = np.ascontiguousarray(inputs9, np.float32)
+        d = np.ascontiguousarray(dirs, np.float32)
+        w = np.ascontiguousarray(weights, np.float32)
+        loss = np.zeros(1, np.float32)
+        self._chk(self.lib.lfh_guide_train_step(self.h, fptr(x), fptr(d), fptr(w), x.shape[0], fptr(loss)))
+        return float(loss[0])
+
+    def steps(self):
+        n = C.c_int64()
+        self._chk(self.lib.lfh_guide_steps(self.h, C.byref(n)))
+        return n.value
 
 
 class Server:

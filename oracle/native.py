@@ -26,7 +26,8 @@ class Options(C.Structure):
                 ("envmap", C.c_int32), ("lights", C.c_int32), ("constant_bg", C.c_int32),
                 ("bg", C.c_float * 3), ("hdr_multiplier", C.c_float),
                 ("cam_pos", C.c_float * 3), ("cam_right", C.c_float * 3), ("cam_up", C.c_float * 3),
-                ("cam_fwd", C.c_float * 3), ("fov", C.c_float), ("focal_dist", C.c_float), ("aperture", C.c_float)]
+                ("cam_fwd", C.c_float * 3), ("fov", C.c_float), ("focal_dist", C.c_float), ("aperture", C.c_float),
+                ("estimator", C.c_int32)]
 
 
 _lib = None
@@ -217,4 +218,5 @@ def options_from(lfh_opts, cam, lights: bool, envmap: bool) -> Options:
     o.cam_up[:] = list(cam.up)
     o.cam_fwd[:] = list(cam.forward)
     o.fov, o.focal_dist, o.aperture = cam.fov, cam.focal_dist, cam.aperture
+    o.estimator = lfh_opts.estimator
     return o

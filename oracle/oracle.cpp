@@ -938,6 +938,7 @@ template <class T> V3<T> PathTrace(const orc_scene* s, const orc_options* o, Ray
     LightSampleRec<T> lrec{};
     BsdfSampleRec<T> brec{};
     V3<T> absorption = mk(T(0), T(0), T(0));
+    T misPdf = T(0);   // density of the BSDF strategy in the MIS weights of the next hit
     M4<T> xf;
     for (int c = 0; c < 4; ++c)
         for (int k = 0; k < 4; ++k) xf.m[c][k] = (c == k) ? T(1) : T(0);
@@ -955,7 +956,7 @@ template <class T> V3<T> PathTrace(const orc_scene* s, const orc_options* o, Ray
                 T v = std::acos(r.direction.y) * (T(1) / C<T>::PI);
                 if (depth > 0) {
                     T lightPdf = EnvPdf<T>(s, r);
-                    misWeight = powerHeuristic(brec.pdf, lightPdf);
+                    misWeight = powerHeuristic(misPdf, lightPdf);
                 }
                 radiance = radiance + misWeight * tex_linear3<T>(s->hcols, s->hw, s->hh, u, v) * throughput *
                                           (T)o->hdr_multiplier;
@@ -967,13 +968,20 @@ template <class T> V3<T> PathTrace(const orc_scene* s, const orc_options* o, Ray
         if (dot(st.normal, st.ffnormal) > T(0)) absorption = mk(T(0), T(0), T(0));
         radiance = radiance + st.mat.emission * throughput;
         if (o->lights && st.isEmitter) {
-            V3<T> Le = (st.depth == 0) ? lrec.emission : powerHeuristic(brec.pdf, lrec.pdf) * lrec.emission;
+            V3<T> Le = (st.depth == 0) ? lrec.emission : powerHeuristic(misPdf, lrec.pdf) * lrec.emission;
             radiance = radiance + Le * throughput;
             break;
         }
         throughput = throughput * mk(std::exp(-absorption.x * t), std::exp(-absorption.y * t), std::exp(-absorption.z * t));
         radiance = radiance + DirectLight(s, o, r, st, seed) * throughput;
         brec.f = DisneySample(st, -r.direction, st.ffnormal, brec.L, brec.pdf, seed);
+        misPdf = brec.pdf;
+        if (o->estimator == 1 && brec.pdf > T(0)) {
+            // consistent MIS: the lobe-mixture pdf of DisneyEval (disney.glsl:289-290)
+            T mp;
+            DisneyEval(st, -r.direction, st.ffnormal, brec.L, mp);
+            misPdf = mp;
+        }
         if (dot(st.ffnormal, brec.L) < T(0))
             absorption = mk(-std::log(st.mat.extinction.x) / st.mat.atDistance, -std::log(st.mat.extinction.y) / st.mat.atDistance,
                             -std::log(st.mat.extinction.z) / st.mat.atDistance);

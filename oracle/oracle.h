@@ -70,6 +70,7 @@ typedef struct orc_options {
     float hdr_multiplier;
     float cam_pos[3], cam_right[3], cam_up[3], cam_fwd[3];
     float fov, focal_dist, aperture;
+    int32_t estimator;   /* 0: the reference's MIS weights; 1: lobe-mixture pdf after BSDF samples */
 } orc_options;
 
 /* Shading frame of the per-stage entry points: 16 floats
