@@ -47,6 +47,13 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=6_000_000, help="rays in the CPU-baseline sample")
     ap.add_argument("--profile-only", action="store_true", help="skip stats / baseline (for rocprof runs)")
+    ap.add_argument("--legs", default="c0,c2,c4",
+                    help="extra path-tracing measurements reported beside the traversal line: c0 (Cornell "
+                         "Msamples/s), c2 (instanced scene Msamples/s and Mrays/s), c4 (serving p50/p99)")
+    ap.add_argument("--guiding", action="store_true",
+                    help="instead: wall time to RMSE 1e-2, guided / plain, on the guiding room (own JSON line)")
+    ap.add_argument("--guide-ref-spp", type=int, default=4096)
+    ap.add_argument("--guide-max-spp", type=int, default=1024)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); rays shard with no exchange "
@@ -58,6 +65,9 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.init()
+    if args.guiding:
+        print(json.dumps(guiding_leg(args)), flush=True)
+        return
     t0 = time.time()
     tris = scenes.soup_triangles(args.tris, seed=0)
     s = lfh.Scene()
@@ -160,6 +170,16 @@ def main():
     cpu = None
     if not args.no_cpu_baseline and not args.profile_only:
         cpu = cpu_baseline(s, rays, n_coh, args.cpu_sample)
+    if not args.profile_only and args.legs:
+        del d_rays, d_hits, d_occ
+        torch.cuda.empty_cache()
+        legs = {"c0": cornell_leg, "c2": instanced_leg, "c4": serving_leg}
+        for leg in args.legs.split(","):
+            try:
+                extra.update(legs[leg.strip()](stream))
+            except Exception as e:  # a failing extra leg must not hide the main measurement
+                log(f"leg {leg} failed: {type(e).__name__}: {e}")
+                extra[f"{leg}_error"] = f"{type(e).__name__}: {e}"
 
     out = {
         "metric": METRIC,
@@ -220,6 +240,230 @@ def cpu_baseline(scene, rays, n_coh, sample):
     return {"value": round(len(sub) / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
             "sample": f"{len(sub)} rays ({half} pinhole + {sample - half} incoherent, evenly strided) of the "
                       "same 16M-ray workload; FP32 restatement of ClosestHit in the reference visit order"}
+
+
+# ---- path-tracing legs ------------------------------------------------------------------
+def _timed_passes(ctx, stream, passes):
+    """Renders `passes` full-frame passes; returns GPU milliseconds (HIP events
+    on the context's stream)."""
+    import torch
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    ctx.render_passes(passes)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1)
+
+
+def cornell_leg(stream):
+    """Cornell box 256 x 256, 64 samples per pixel, max depth 4 (one tile)."""
+    from lavaframe_amd import lfh, scenes
+    sc = scenes.cornell(256, 256, max_depth=4)
+    ctx = lfh.Context.from_scene(sc)
+    ctx.set_stream(stream.cuda_stream)
+    cam = sc.camera()
+    ctx.update(cam)
+    _timed_passes(ctx, stream, 4)      # warm-up
+    ctx.update(cam, reset=True)
+    ctx.update(cam)
+    r0 = ctx.ray_counts()
+    spp = 64
+    ms = _timed_passes(ctx, stream, spp)
+    r1 = ctx.ray_counts()
+    samples = 256 * 256 * spp
+    rays = (r1[0] - r0[0]) + (r1[1] - r0[1])
+    log(f"cornell 256^2 x {spp} spp: {ms:.2f} ms, {samples / ms / 1e3:.1f} Msamples/s, {rays / ms / 1e3:.1f} Mrays/s")
+    return {"cornell_c0": {"msamples_per_s": round(samples / (ms * 1e-3) / 1e6, 1),
+                           "mrays_per_s": round(rays / (ms * 1e-3) / 1e6, 1), "spp": spp,
+                           "resolution": [256, 256], "max_depth": 4, "ms": round(ms, 2)}}
+
+
+def instanced_leg(stream):
+    """64 procedural meshes (1k-200k triangles), 1024 random affine instances,
+    random Disney materials, 8 quad lights + 2048 x 1024 sky; 1920 x 1080,
+    max depth 8; 16 samples per pixel timed."""
+    from lavaframe_amd import lfh, scenes
+    t = time.time()
+    sc = scenes.instanced()
+    ntri = int(sc.arrays()["vert_indices"].shape[0])
+    log(f"instanced scene: {ntri} triangles, build {time.time() - t:.1f}s")
+    ctx = lfh.Context.from_scene(sc)
+    ctx.set_stream(stream.cuda_stream)
+    cam = sc.camera()
+    ctx.update(cam)
+    _timed_passes(ctx, stream, 2)
+    r0 = ctx.ray_counts()
+    spp = 16
+    ms = _timed_passes(ctx, stream, spp)
+    r1 = ctx.ray_counts()
+    samples = 1920 * 1080 * spp
+    ext, sh = r1[0] - r0[0], r1[1] - r0[1]
+    log(f"instanced 1920x1080 x {spp} spp: {ms:.1f} ms, {samples / ms / 1e3:.1f} Msamples/s, "
+        f"{(ext + sh) / ms / 1e3:.1f} Mrays/s")
+    return {"instanced_c2": {"msamples_per_s": round(samples / (ms * 1e-3) / 1e6, 1),
+                             "mrays_per_s": round((ext + sh) / (ms * 1e-3) / 1e6, 1),
+                             "extend_rays": ext, "shadow_rays": sh, "spp": spp, "triangles": ntri,
+                             "instances": 1024, "resolution": [1920, 1080], "max_depth": 8, "ms": round(ms, 1)}}
+
+
+def _job_stream(n_jobs, rate, seed):
+    """Poisson arrivals (rate jobs/s) of jobs drawn from three scene families
+    with random resolutions and sample targets."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, n_jobs))
+    res = [(256, 256), (512, 512), (640, 360), (960, 540)]
+    spp = [16, 32, 64, 128]
+    jobs = []
+    for i in range(n_jobs):
+        r = res[rng.choice(4, p=[0.4, 0.3, 0.2, 0.1])]
+        jobs.append({"t": float(t[i]), "family": int(rng.integers(0, 3)), "res": r,
+                     "spp": spp[rng.choice(4, p=[0.4, 0.3, 0.2, 0.1])]})
+    return jobs
+
+
+def _run_stream(scn, jobs, max_jobs):
+    from lavaframe_amd import lfh
+    import torch
+    srv = lfh.Server(budget_bytes=0, max_jobs=max_jobs)
+    ids = {}
+    t0 = time.perf_counter()
+    nxt = 0
+    while True:
+        now = time.perf_counter() - t0
+        while nxt < len(jobs) and jobs[nxt]["t"] <= now:
+            j = jobs[nxt]
+            w, h = j["res"]
+            sc = scn[j["family"]]
+            ids[nxt] = (srv.submit_scene(sc, j["spp"], width=w, height=h, tile_width=w, tile_height=h),
+                        time.perf_counter() - t0)
+            nxt += 1
+        active = srv.step()
+        if active == 0:
+            if nxt >= len(jobs):
+                break
+            time.sleep(max(0.0, jobs[nxt]["t"] - (time.perf_counter() - t0)))
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t0
+    lat, samples = [], 0
+    for k, (jid, ts) in ids.items():
+        st = srv.status(jid)
+        lat.append(st.t_done - st.t_submit)
+        w, h = jobs[k]["res"]
+        samples += w * h * jobs[k]["spp"]
+    srv.close()
+    lat = np.array(lat)
+    return {"p50_s": round(float(np.percentile(lat, 50)), 4), "p99_s": round(float(np.percentile(lat, 99)), 4),
+            "aggregate_msamples_per_s": round(samples / total / 1e6, 1), "makespan_s": round(total, 3)}
+
+
+def serving_leg(stream):
+    """Render serving: 60 jobs, Poisson arrivals at 20 jobs/s, from the Cornell,
+    instanced (small) and guiding-room generators at 256^2-960x540 and 16-128
+    spp; latency = submit to target spp.  Baseline: the same stream with one
+    job admitted at a time."""
+    from lavaframe_amd import scenes
+    scn = [scenes.cornell(256, 256), scenes.instanced(n_meshes=16, n_instances=128, max_tris=50000,
+                                                      hdr_size=(512, 256)),
+           scenes.aperture_room(target_tris=50000, max_depth=6)]
+    jobs = _job_stream(60, 20.0, seed=0)
+    batched = _run_stream(scn, jobs, 64)
+    serial = _run_stream(scn, jobs, 1)
+    log(f"serving: batched {batched}, serial {serial}")
+    return {"serving_c4": {"jobs": len(jobs), "rate_jobs_per_s": 20.0, "batched": batched, "serial_baseline": serial}}
+
+
+def _rel_rmse(img, ref):
+    return float(np.sqrt(np.mean((img - ref) ** 2)) / max(float(ref.mean()), 1e-12))
+
+
+def _converge(ctx, cam, ref, stream, max_spp, target, guide=None):
+    """Renders 1, 2, 4, ... spp checkpoints; returns [(spp, seconds, rel-RMSE)]
+    with the wall clock paused while the image is read back."""
+    import torch
+    ctx.update(cam, reset=True)   # sample counter restarts; render_passes advances it per pass
+    pts, spent, spp = [], 0.0, 0
+    step = 1
+    while spp < max_spp:
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        ctx.render_passes(step)
+        torch.cuda.synchronize()
+        spent += time.perf_counter() - t
+        spp += step
+        img = ctx.output_hdr()
+        e = _rel_rmse(img, ref)
+        pts.append((spp, spent, e))
+        if e < target:
+            break
+        step = spp
+    return pts
+
+
+def _time_to(pts, target):
+    """Wall time at which rel-RMSE reaches target: log-log interpolation between
+    checkpoints, or extrapolation with RMSE ~ t^-1/2 past the last one."""
+    for (s0, t0, e0), (s1, t1, e1) in zip(pts, pts[1:]):
+        if e1 <= target <= e0:
+            a = (math.log(target) - math.log(e0)) / (math.log(e1) - math.log(e0))
+            return math.exp(math.log(t0) + a * (math.log(t1) - math.log(t0))), False
+    s, t, e = pts[-1]
+    return t * (e / target) ** 2, e > target
+
+
+def guiding_leg(args):
+    """Guiding room (about 200k triangles, light seen only through a door of 5 %
+    of a wall), 1280 x 720, max depth 10.  Both runs use the consistent
+    estimator (options.estimator = 1), which is unbiased; the reference image is
+    the plain run at --guide-ref-spp.  Error = relative RMSE (RMS pixel error /
+    mean reference radiance).  Guided: one online Adam step per 65536 recorded
+    samples after every pass."""
+    import torch
+    from lavaframe_amd import lfh, scenes
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
+    sc = scenes.aperture_room()
+    cam = sc.camera()
+    target = 1e-2
+    ref_ctx = lfh.Context.from_scene(sc, estimator=1)
+    ref_ctx.set_stream(stream.cuda_stream)
+    ref_ctx.update(cam, reset=True)
+    t = time.perf_counter()
+    done = 0
+    while done < args.guide_ref_spp:
+        k = min(64, args.guide_ref_spp - done)
+        ref_ctx.render_passes(k)
+        done += k
+        torch.cuda.synchronize()
+        log(f"reference {done} spp ({time.perf_counter() - t:.1f}s)")
+    ref = ref_ctx.output_hdr()
+    t_ref = time.perf_counter() - t
+    ref_ctx.close()
+    plain_ctx = lfh.Context.from_scene(sc, estimator=1)
+    plain_ctx.set_stream(stream.cuda_stream)
+    plain = _converge(plain_ctx, cam, ref, stream, args.guide_max_spp, target)
+    log(f"plain: {plain}")
+    plain_ctx.close()
+    g_ctx = lfh.Context.from_scene(sc, estimator=1, guiding=1)
+    g_ctx.set_stream(stream.cuda_stream)
+    guide = lfh.Guide.for_scene(sc, seed=0)
+    g_ctx.set_guide(guide, train_every=1)
+    guided = _converge(g_ctx, cam, ref, stream, args.guide_max_spp, target)
+    log(f"guided: {guided} (Adam steps {guide.steps()}, samples {g_ctx.guide_samples_seen()})")
+    tp, xp = _time_to(plain, target)
+    tg, xg = _time_to(guided, target)
+    # the reference's own noise floor at the target
+    ref_noise = target * math.sqrt(plain[-1][0] / done) if not xp else None
+    return {"metric": "wall time to rel-RMSE 1e-2, guided / plain (consistent estimator)",
+            "value": round(tg / tp, 4), "unit": "ratio", "higher_is_better": False, "n_gpus": 1,
+            "guided_s": round(tg, 3), "plain_s": round(tp, 3), "guided_extrapolated": xg, "plain_extrapolated": xp,
+            "plain_curve": [[s, round(t, 4), round(e, 5)] for s, t, e in plain],
+            "guided_curve": [[s, round(t, 4), round(e, 5)] for s, t, e in guided],
+            "reference": {"spp": done, "seconds": round(t_ref, 1), "noise_at_target_spp": ref_noise},
+            "adam_steps": guide.steps(), "training_samples": g_ctx.guide_samples_seen(),
+            "dtype": "f32 (MLP bf16/f32-acc)", "data": "synthetic",
+            "config": {"workload": "guiding room ~200k triangles, 1280x720, depth 10, door 5% of wall",
+                       "triangles": int(sc.arrays()["vert_indices"].shape[0])}}
 
 
 if __name__ == "__main__":

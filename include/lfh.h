@@ -207,6 +207,10 @@ int  lfh_update(lfh_ctx* ctx, const lfh_camera* cam, int reset);
  * and accumulates it. */
 int  lfh_render(lfh_ctx* ctx);
 int  lfh_get_sample_count(const lfh_ctx* ctx, int32_t* out);
+/* Rays traced by this ctx's path tracer so far: closest-hit (camera and
+ * bounce) rays and shadow rays.  No reference counterpart (the reference keeps
+ * no counters); used for rays/s figures. */
+int  lfh_get_ray_counts(lfh_ctx* ctx, int64_t* extended, int64_t* shadow);
 int  lfh_get_frame(const lfh_ctx* ctx, int32_t* out);
 int  lfh_get_progress(const lfh_ctx* ctx, float* out);
 /* Linear HDR accum / samples, RGB32F, bottom-up rows like GL; dst is caller-

@@ -181,6 +181,17 @@ int lfh_render_passes(lfh_ctx* c, int32_t passes) {
     return LFH_OK;
 }
 
+int lfh_get_ray_counts(lfh_ctx* c, int64_t* extended, int64_t* shadow) {
+    if (!c || !extended || !shadow) return rfail(c, LFH_E_INVALID, "lfh_get_ray_counts: NULL argument");
+    unsigned long long t[2] = {0, 0};
+    RCHK(c, hipMemcpyAsync(t, c->pass.work.as<unsigned long long>() + 8, sizeof t, hipMemcpyDeviceToHost, c->stream),
+         "lfh_get_ray_counts");
+    RCHK(c, hipStreamSynchronize(c->stream), "lfh_get_ray_counts");
+    *extended = (int64_t)t[0];
+    *shadow = (int64_t)t[1];
+    return LFH_OK;
+}
+
 int lfh_get_sample_count(const lfh_ctx* c, int32_t* out) {
     if (!c || !out) return rfail(nullptr, LFH_E_INVALID, "lfh_get_sample_count: NULL argument");
     *out = c->sampleCounter;

@@ -49,8 +49,12 @@ __global__ void k_pt_reset(uint32_t* counters, unsigned long long* work) {
     if (i < 8) work[i] = 0ull;
 }
 
+// work[0..7]: per-launch fetch counters; work[8], work[9]: running totals of
+// extended and shadow rays (never reset by a pass).
 __global__ void k_pt_advance(uint32_t* consumed, uint32_t* shadow_count, unsigned long long* work) {
     if (threadIdx.x == 0) {
+        work[8] += *consumed;
+        work[9] += *shadow_count;
         *consumed = 0u;
         *shadow_count = 0u;
     }

@@ -149,6 +149,7 @@ _SIGS = {
     "lfh_guide_train_step": (C.c_int, [_P, f32p, f32p, f32p, C.c_int64, f32p]),
     "lfh_set_guide": (C.c_int, [_P, _P, C.c_int32]),
     "lfh_guide_steps": (C.c_int, [_P, i64p]),
+    "lfh_get_ray_counts": (C.c_int, [_P, i64p, i64p]),
     "lfh_guide_sample": (C.c_int, [_P, f32p, u32p, C.c_int64, f32p, f32p]),
     "lfh_guide_pdf": (C.c_int, [_P, f32p, f32p, C.c_int64, f32p]),
     "lfh_guide_samples_seen": (C.c_int, [_P, i64p]),
@@ -555,6 +556,11 @@ class Context:
         pdf = np.zeros(d.shape[0], np.float32)
         self._chk(self.lib.lfh_env_pdf(self.h, fptr(d), d.shape[0], fptr(pdf)))
         return pdf
+
+    def ray_counts(self):
+        a, b = C.c_int64(), C.c_int64()
+        self._chk(self.lib.lfh_get_ray_counts(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def set_guide(self, guide: "Guide | None", train_every: int = 1):
         self._guide = guide  # keep the guide alive while attached
