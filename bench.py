@@ -52,8 +52,8 @@ def main():
                          "Msamples/s), c2 (instanced scene Msamples/s and Mrays/s), c4 (serving p50/p99)")
     ap.add_argument("--guiding", action="store_true",
                     help="instead: wall time to RMSE 1e-2, guided / plain, on the guiding room (own JSON line)")
-    ap.add_argument("--guide-ref-spp", type=int, default=4096)
-    ap.add_argument("--guide-max-spp", type=int, default=1024)
+    ap.add_argument("--guide-ref-spp", type=int, default=2048, help="spp of each reference half")
+    ap.add_argument("--guide-max-spp", type=int, default=4096)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); rays shard with no exchange "
@@ -373,12 +373,28 @@ def serving_leg(stream):
     return {"serving_c4": {"jobs": len(jobs), "rate_jobs_per_s": 20.0, "batched": batched, "serial_baseline": serial}}
 
 
-def _rel_rmse(img, ref):
-    return float(np.sqrt(np.mean((img - ref) ** 2)) / max(float(ref.mean()), 1e-12))
+BLOCK = 16
 
 
-def _converge(ctx, cam, ref, stream, max_spp, target, guide=None):
-    """Renders 1, 2, 4, ... spp checkpoints; returns [(spp, seconds, rel-RMSE)]
+def _blocks(a):
+    """Means of BLOCK x BLOCK pixel blocks of an (H, W, 3) image."""
+    H, W = a.shape[0] // BLOCK * BLOCK, a.shape[1] // BLOCK * BLOCK
+    return a[:H, :W].reshape(H // BLOCK, BLOCK, W // BLOCK, BLOCK, 3).mean((1, 3))
+
+
+def _rmse(img, ref, ref_var):
+    """Relative RMSE of the block means against a noisy reference (block
+    means), with the reference's own variance removed:
+    E[(img - ref)^2] = MSE(img) + Var(ref); normalised by the mean radiance."""
+    d = float(np.mean((_blocks(img) - ref) ** 2)) - ref_var
+    return float(np.sqrt(max(d, 0.0))) / ref_mean_holder[0]
+
+
+ref_mean_holder = [1.0]
+
+
+def _converge(ctx, cam, ref, ref_var, stream, max_spp, target):
+    """Renders 1, 2, 4, ... spp checkpoints; returns [(spp, seconds, RMSE)]
     with the wall clock paused while the image is read back."""
     import torch
     ctx.update(cam, reset=True)   # sample counter restarts; render_passes advances it per pass
@@ -392,7 +408,7 @@ def _converge(ctx, cam, ref, stream, max_spp, target, guide=None):
         spent += time.perf_counter() - t
         spp += step
         img = ctx.output_hdr()
-        e = _rel_rmse(img, ref)
+        e = _rmse(img, ref, ref_var)
         pts.append((spp, spent, e))
         if e < target:
             break
@@ -414,10 +430,12 @@ def _time_to(pts, target):
 def guiding_leg(args):
     """Guiding room (about 200k triangles, light seen only through a door of 5 %
     of a wall), 1280 x 720, max depth 10.  Both runs use the consistent
-    estimator (options.estimator = 1), which is unbiased; the reference image is
-    the plain run at --guide-ref-spp.  Error = relative RMSE (RMS pixel error /
-    mean reference radiance).  Guided: one online Adam step per 65536 recorded
-    samples after every pass."""
+    estimator (options.estimator = 1), which is unbiased; the reference is the
+    plain estimator at 2 x --guide-ref-spp samples from a disjoint frame range,
+    rendered as two halves whose difference gives the reference's own
+    variance.  Error = RMSE of the 16 x 16-pixel block means (80 x 45 blocks)
+    with that variance removed, relative to the mean radiance.  Guided: online
+    Adam steps on the samples recorded from every path after each pass."""
     import torch
     from lavaframe_amd import lfh, scenes
     stream = torch.cuda.Stream()
@@ -427,44 +445,53 @@ def guiding_leg(args):
     target = 1e-2
     ref_ctx = lfh.Context.from_scene(sc, estimator=1)
     ref_ctx.set_stream(stream.cuda_stream)
+    ref_ctx.set_frame_offset(1 << 24)
     ref_ctx.update(cam, reset=True)
     t = time.perf_counter()
-    done = 0
-    while done < args.guide_ref_spp:
-        k = min(64, args.guide_ref_spp - done)
-        ref_ctx.render_passes(k)
-        done += k
-        torch.cuda.synchronize()
-        log(f"reference {done} spp ({time.perf_counter() - t:.1f}s)")
-    ref = ref_ctx.output_hdr()
+    half = args.guide_ref_spp
+    halves = []
+    for h in range(2):
+        done = 0
+        while done < half:
+            k = min(64, half - done)
+            ref_ctx.render_passes(k)
+            done += k
+            torch.cuda.synchronize()
+            log(f"reference half {h}: {done} spp ({time.perf_counter() - t:.1f}s)")
+        s_, c_ = ref_ctx.accum()
+        halves.append((s_ - c_).astype(np.float64))
+    h1 = _blocks(halves[0] / half)
+    h2 = _blocks((halves[1] - halves[0]) / half)
+    ref = 0.5 * (h1 + h2)
+    ref_var = float(np.mean((h1 - h2) ** 2)) / 4.0
+    ref_mean_holder[0] = float(ref.mean())
     t_ref = time.perf_counter() - t
     ref_ctx.close()
+    log(f"reference: mean radiance {ref.mean():.5f}, block noise RMS {math.sqrt(ref_var) / ref.mean():.4g} (relative)")
     plain_ctx = lfh.Context.from_scene(sc, estimator=1)
     plain_ctx.set_stream(stream.cuda_stream)
-    plain = _converge(plain_ctx, cam, ref, stream, args.guide_max_spp, target)
+    plain = _converge(plain_ctx, cam, ref, ref_var, stream, args.guide_max_spp, target)
     log(f"plain: {plain}")
     plain_ctx.close()
     g_ctx = lfh.Context.from_scene(sc, estimator=1, guiding=1)
     g_ctx.set_stream(stream.cuda_stream)
     guide = lfh.Guide.for_scene(sc, seed=0)
     g_ctx.set_guide(guide, train_every=1)
-    guided = _converge(g_ctx, cam, ref, stream, args.guide_max_spp, target)
+    guided = _converge(g_ctx, cam, ref, ref_var, stream, args.guide_max_spp, target)
     log(f"guided: {guided} (Adam steps {guide.steps()}, samples {g_ctx.guide_samples_seen()})")
     tp, xp = _time_to(plain, target)
     tg, xg = _time_to(guided, target)
-    # the reference's own noise floor at the target
-    ref_noise = target * math.sqrt(plain[-1][0] / done) if not xp else None
-    return {"metric": "wall time to rel-RMSE 1e-2, guided / plain (consistent estimator)",
+    return {"metric": "wall time to relative RMSE 1e-2 (16x16 block means), guided / plain (consistent estimator)",
             "value": round(tg / tp, 4), "unit": "ratio", "higher_is_better": False, "n_gpus": 1,
             "guided_s": round(tg, 3), "plain_s": round(tp, 3), "guided_extrapolated": xg, "plain_extrapolated": xp,
-            "plain_curve": [[s, round(t, 4), round(e, 5)] for s, t, e in plain],
-            "guided_curve": [[s, round(t, 4), round(e, 5)] for s, t, e in guided],
-            "reference": {"spp": done, "seconds": round(t_ref, 1), "noise_at_target_spp": ref_noise},
+            "plain_curve": [[s, round(t, 4), round(e, 6)] for s, t, e in plain],
+            "guided_curve": [[s, round(t, 4), round(e, 6)] for s, t, e in guided],
+            "reference": {"spp": 2 * half, "seconds": round(t_ref, 1), "mean_radiance": round(float(ref.mean()), 5),
+                          "noise_rms": round(math.sqrt(ref_var), 6)},
             "adam_steps": guide.steps(), "training_samples": g_ctx.guide_samples_seen(),
             "dtype": "f32 (MLP bf16/f32-acc)", "data": "synthetic",
             "config": {"workload": "guiding room ~200k triangles, 1280x720, depth 10, door 5% of wall",
                        "triangles": int(sc.arrays()["vert_indices"].shape[0])}}
-
 
 if __name__ == "__main__":
     main()

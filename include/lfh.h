@@ -207,6 +207,12 @@ int  lfh_update(lfh_ctx* ctx, const lfh_camera* cam, int reset);
  * and accumulates it. */
 int  lfh_render(lfh_ctx* ctx);
 int  lfh_get_sample_count(const lfh_ctx* ctx, int32_t* out);
+/* Offset added to the frame number that seeds each pass's random numbers
+ * (InitRNG's frame, globals.glsl:116-120; default 0 = the reference's
+ * sequence).  Contexts with disjoint frame ranges render independent samples
+ * of the same image, e.g. one sample-index shard per GPU.  No reference
+ * counterpart. */
+int  lfh_set_frame_offset(lfh_ctx* ctx, int32_t offset);
 /* Rays traced by this ctx's path tracer so far: closest-hit (camera and
  * bounce) rays and shadow rays.  No reference counterpart (the reference keeps
  * no counters); used for rays/s figures. */

@@ -123,8 +123,8 @@ int lfh_update_instances(lfh_ctx* c, const float* transforms, const float* mater
     auto cp = [&](DevBuf& b, const std::vector<float>& v) {
         return hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(float), hipMemcpyHostToDevice, c->stream);
     };
-    HIPCHK(c, cp(c->ds.nodes, c->ps.nodes), "lfh_update_instances: nodes");
-    HIPCHK(c, cp(c->ds.insts, c->ps.insts), "lfh_update_instances: instances");
+    HIPCHK(c, cp(c->ds.nodes, c->ps.nodes4), "lfh_update_instances: nodes");
+    HIPCHK(c, cp(c->ds.insts, c->ps.insts4), "lfh_update_instances: instances");
     HIPCHK(c, cp(c->ds.xforms, c->ps.xforms), "lfh_update_instances: transforms");
     HIPCHK(c, cp(c->ds.nmats, c->ps.nmats), "lfh_update_instances: normal matrices");
     HIPCHK(c, cp(c->ds.materials, c->ps.materials), "lfh_update_instances: materials");
@@ -138,7 +138,7 @@ static int check_err_flag(lfh_ctx* c) {
     HIPCHK(c, hipStreamSynchronize(c->stream), "trace: flag");
     if (flag & 1) {
         hipMemsetAsync(c->errflag.p, 0, 64, c->stream);
-        return ctx_fail(c, LFH_E_UNSUPPORTED, "traversal stack overflow (BVH deeper than 96 levels)");
+        return ctx_fail(c, LFH_E_UNSUPPORTED, "traversal stack overflow");
     }
     return LFH_OK;
 }

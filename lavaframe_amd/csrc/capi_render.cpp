@@ -2,6 +2,7 @@
 // output) and the per-stage parity entry points.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -79,9 +80,16 @@ int render_init(lfh_ctx* c) {
 
 void render_release(lfh_ctx*) {}
 
+// Paths per wavefront targeted by multi-sample passes over small frames.
+constexpr int64_t kBatchPaths = 1 << 21;
+constexpr int kMaxBatch = 64;
+
 // One path-traced pass over the current tile (TiledRenderer::Render's
-// pathTraceShader + outputShader draws, TiledRenderer.cpp:334-344).
-int render_tile(lfh_ctx* c) {
+// pathTraceShader + outputShader draws, TiledRenderer.cpp:334-344); batch > 1
+// renders that many consecutive passes of a one-tile frame in one wavefront
+// (frames frameCounter .. frameCounter + batch - 1), accumulated per pixel in
+// frame order.
+int render_tile(lfh_ctx* c, int batch) {
     lfhd::CamParams cp;
     std::memcpy(cp.pos, c->cam.position, 12);
     std::memcpy(cp.right, c->cam.right, 12);
@@ -96,7 +104,16 @@ int render_tile(lfh_ctx* c) {
     cp.tile_h = c->tileH;
     cp.tileX = c->tileX;
     cp.tileY = c->tileY;
-    cp.frame = c->frameCounter;
+    cp.frame = c->frameCounter + c->frameOffset;
+    cp.batch = batch;
+    if (batch > 1) {
+        const int64_t need = (int64_t)c->rs.npaths * batch;
+        if (c->pass.cap < need) {
+            hipError_t e = hipStreamSynchronize(c->stream);
+            if (e == hipSuccess) e = c->pass.alloc((int)need);
+            if (e != hipSuccess) return rhip(c, e, "render: batch buffers");
+        }
+    }
     PassBufs& pb = c->pass;
     lfhd::ShadeScene sc = shade_view(c->ps, c->ds, c->opt);
     uint32_t* cnt = pb.counters.as<uint32_t>();
@@ -164,20 +181,44 @@ int lfh_render(lfh_ctx* c) {
     if (!c) return rfail(c, LFH_E_INVALID, "lfh_render: NULL ctx");
     if (!c->has_cam) return rfail(c, LFH_E_STATE, "lfh_render: lfh_update has not been called");
     if (c->tileX < 0) return LFH_OK;  // after a reset the reference draws only the preview
-    return lfh::render_tile(c);
+    return lfh::render_tile(c, 1);
 }
 
 int lfh_render_passes(lfh_ctx* c, int32_t passes) {
     if (!c || passes < 0) return rfail(c, LFH_E_INVALID, "lfh_render_passes: bad argument");
     if (!c->has_cam) return rfail(c, LFH_E_STATE, "lfh_render_passes: lfh_update has not been called");
     const int tiles = c->numTilesX * c->numTilesY;
+    if (tiles == 1 && !c->guide) {
+        // one-tile frames: consecutive passes share one wavefront
+        int kmax = (int)std::min<int64_t>(lfh::kMaxBatch, std::max<int64_t>(1, lfh::kBatchPaths / c->rs.npaths));
+        int done = 0;
+        while (done < passes) {
+            const int k = std::min(kmax, passes - done);
+            lfh::advance_counters(c, false);
+            const int frame0 = c->frameCounter;
+            for (int i = 1; i < k; ++i) lfh::advance_counters(c, false);  // frames frame0 .. frame0 + k - 1
+            const int keep = c->frameCounter;
+            c->frameCounter = frame0;
+            int r = lfh::render_tile(c, k);
+            c->frameCounter = keep;
+            if (r != LFH_OK) return r;
+            done += k;
+        }
+        return LFH_OK;
+    }
     for (int p = 0; p < passes; ++p) {
         for (int t = 0; t < tiles; ++t) {
             lfh::advance_counters(c, false);
-            int r = lfh::render_tile(c);
+            int r = lfh::render_tile(c, 1);
             if (r != LFH_OK) return r;
         }
     }
+    return LFH_OK;
+}
+
+int lfh_set_frame_offset(lfh_ctx* c, int32_t offset) {
+    if (!c || offset < 0) return rfail(c, LFH_E_INVALID, "lfh_set_frame_offset: bad argument");
+    c->frameOffset = offset;
     return LFH_OK;
 }
 

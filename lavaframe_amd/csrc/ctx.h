@@ -37,7 +37,11 @@ struct DevBuf {
 
 // Host-side packed copy of a scene (built from an lfh_scene_desc).
 struct PackedScene {
-    std::vector<float> nodes;     // 16 floats / internal node
+    std::vector<float> nodes;     // 16 floats / internal BVH2 node (host intermediate)
+    std::vector<float> nodes4;    // 32 floats / BVH4 node (device layout)
+    std::vector<float> insts4;    // insts with BLAS refs into nodes4 (device layout)
+    uint32_t root4 = lfhd::kRefSentinel;
+    int stack_need = 0;           // worst-case traversal stack entries of nodes4
     std::vector<float> tris;      // 12 floats / triangle slot
     std::vector<float> insts;     // 16 floats / instance (inverse rows + meta)
     std::vector<float> xforms;    // 16 floats / instance (forward, column-major)
@@ -62,6 +66,9 @@ struct PackedScene {
 
 // Build the device layout from the reference flat arrays; "" on success.
 std::string pack_scene(const lfh_scene_desc& d, PackedScene* out);
+// BVH4 device nodes from the packed BVH2 (called by pack_scene and after
+// instance updates); "" on success.
+std::string widen_bvh(PackedScene* ps);
 // Re-pack the instance-dependent parts after Renderer::Update-style edits.
 std::string repack_instances(PackedScene* ps, const float* transforms, const float* tlas_nodes,
                              int64_t first_node, int64_t count, const float* materials);
@@ -123,7 +130,8 @@ struct GuideBufs {
     DevBuf lobes;                  // kGuideLobeF4 float4 per hit-queue entry
     DevBuf rec;                    // training records, 4 float4 per slot
     DevBuf samples;                // compacted training samples, 4 float4 each
-    DevBuf block_counts, block_offsets, total, partial;
+    DevBuf block_counts, block_offsets, block_wsum, partial;
+    DevBuf total;                  // int sample count, then float weight sum
     int stride = 1;                // record paths p with p % stride == phase
     int nslots = 0;                // recorded paths x max_depth
 };
@@ -158,6 +166,7 @@ struct lfh_ctx {
     // TiledRenderer counters (TiledRenderer.cpp:48-66, 471-501)
     int tileX = -1, tileY = 0, numTilesX = 1, numTilesY = 1, tileW = 0, tileH = 0;
     int sampleCounter = 1, frameCounter = 1;
+    int frameOffset = 0;           // added to the frame number that seeds the RNG
     // guiding
     struct lfh_guide* guide = nullptr;
     int guide_train_every = 0;

@@ -34,16 +34,30 @@ LFH_HD uint32_t make_leaf(uint32_t first, uint32_t count) { return kRefLeafBit |
 LFH_HD uint32_t make_inst(uint32_t inst) { return kRefInstBit | inst; }
 
 // ---- packed layouts ------------------------------------------------------------
-// Internal BVH2 node, 64 B, both child boxes in the parent:
-//   f4[0] = (lmin.x, lmin.y, lmin.z, lmax.x)
-//   f4[1] = (lmax.y, lmax.z, rmin.x, rmin.y)
-//   f4[2] = (rmin.z, rmax.x, rmax.y, rmax.z)
-//   f4[3] = (as_float(left ref), as_float(right ref), 0, 0)
+// Internal BVH4 node, 128 B (one L2 line), the four child boxes in the parent
+// as structure of arrays, child k in lane k of each float4:
+//   f4[0..2] = min.x, min.y, min.z    f4[3..5] = max.x, max.y, max.z
+//   f4[6]    = as_float(child refs)   f4[7]    = 0
+//   an absent child has ref kRefSentinel.
+// (Host intermediate: the BVH2 node of 64 B with both child boxes,
+//  (lmin, lmax, rmin, rmax, left ref, right ref), collapsed into BVH4 nodes.)
 // Triangle, 48 B, in vert_indices slot order:
 //   f4[0] = (v0.xyz, as_float(prim)),  f4[1] = (e0 = v1 - v0, as_float(slot)),
 //   f4[2] = (e1 = v2 - v0, 0)
 // Instance, 64 B: rows 0..2 of inverse(M) as (inv[0][r], inv[1][r], inv[2][r],
 //   inv[3][r]) and (as_float(blas ref), as_float(material), as_float(index), 0).
+
+// Traversal stack: the newest kLdsStack entries of a lane live in LDS, older
+// ones spill to a per-lane global area of kSpillStride entries.
+#ifndef LFH_LDS_STACK
+#define LFH_LDS_STACK 16
+#endif
+#ifndef LFH_POP_CULL
+#define LFH_POP_CULL 0   // 1: keep box entry distances on the stack and drop stale entries at pop
+#endif
+constexpr int kLdsStack = LFH_LDS_STACK;  // LDS entries per lane (power of 2)
+constexpr int kTraceBlock = 256;        // threads per traversal block (4 wave64)
+constexpr uint32_t kSpillStride = 128;  // max stack entries per lane
 
 struct TraceScene {
     const float4* nodes;

@@ -51,27 +51,39 @@ LFH_HD float guide_norm_coord(float p, float lo, float inv_ext) {
 }
 
 #if defined(__HIP__)
-// Feature f of the encoding (x already normalised).
-LFH_DEV float guide_feature(int f, V3 x, V3 n, V3 wo) {
-    if (f >= 63) return 1.0f;
-    if (f >= 60) return f == 60 ? wo.x : (f == 61 ? wo.y : wo.z);
-    float val;
-    int a, fr, sc;
-    if (f < 36) {
-        a = f / 12;
-        fr = (f % 12) >> 1;
-        sc = f & 1;
-        val = a == 0 ? x.x : (a == 1 ? x.y : x.z);
-    } else {
-        const int g = f < 48 ? f - 36 : f - 48;
-        const V3 s = f < 48 ? n : wo;
-        a = g >> 2;
-        fr = (g >> 1) & 1;
-        sc = g & 1;
-        val = a == 0 ? s.x : (a == 1 ? s.y : s.z);
+// The 64 features of one query (x already normalised).  Each coordinate's
+// base angle pi*c gets one sincos; the higher octaves follow by the double-
+// angle identities sin 2a = 2 sin a cos a, cos 2a = (cos a - sin a)(cos a + sin a),
+// which keeps the absolute error below 1e-5 at the top octave (2^5).
+LFH_DEV void guide_encode(V3 x, V3 n, V3 wo, float* f) {
+    const float xs[3] = {x.x, x.y, x.z}, ns[3] = {n.x, n.y, n.z}, ws[3] = {wo.x, wo.y, wo.z};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float sn, cs;
+        sincosf(kPi * xs[a], &sn, &cs);
+#pragma unroll
+        for (int fr = 0; fr < 6; ++fr) {
+            f[(a * 6 + fr) * 2] = sn;
+            f[(a * 6 + fr) * 2 + 1] = cs;
+            const float s2 = 2.0f * sn * cs, c2 = (cs - sn) * (cs + sn);
+            sn = s2;
+            cs = c2;
+        }
+        sincosf(kPi * ns[a], &sn, &cs);
+        f[36 + a * 4 + 0] = sn;
+        f[36 + a * 4 + 1] = cs;
+        f[36 + a * 4 + 2] = 2.0f * sn * cs;
+        f[36 + a * 4 + 3] = (cs - sn) * (cs + sn);
+        sincosf(kPi * ws[a], &sn, &cs);
+        f[48 + a * 4 + 0] = sn;
+        f[48 + a * 4 + 1] = cs;
+        f[48 + a * 4 + 2] = 2.0f * sn * cs;
+        f[48 + a * 4 + 3] = (cs - sn) * (cs + sn);
     }
-    const float ang = (float)(1 << fr) * kPi * val;
-    return sc ? cosf(ang) : sinf(ang);
+    f[60] = wo.x;
+    f[61] = wo.y;
+    f[62] = wo.z;
+    f[63] = 1.0f;
 }
 
 LFH_DEV uint16_t to_bf16(float v) {
@@ -108,7 +120,19 @@ LFH_DEV void guide_lobes(const float* o, float4* out) {
     }
     out[8] = make_float4(a[0] * inv, a[1] * inv, a[2] * inv, a[3] * inv);
     out[9] = make_float4(a[4] * inv, a[5] * inv, a[6] * inv, a[7] * inv);
+    // a query whose outputs are not all finite gets no guide: weights -1 mark it
+    bool ok = isfinite(inv);
+#pragma unroll
+    for (int k = 0; k < kGuideLobes; ++k)
+        ok = ok && isfinite(out[k].x) && isfinite(out[k].y) && isfinite(out[k].z) && isfinite(out[k].w);
+    if (!ok) {
+        out[8] = make_float4(-1.f, -1.f, -1.f, -1.f);
+        out[9] = make_float4(-1.f, -1.f, -1.f, -1.f);
+    }
 }
+
+// True when the processed lobes of a query are usable (see guide_lobes).
+LFH_DEV bool guide_valid(const float4* __restrict__ lob) { return lob[8].x >= 0.0f; }
 
 // Stores master parameter j (flat layout) into the BF16 operand copies:
 // wf[layer][out][in], wb[layer][in][out], or the FP32 bias.

@@ -3,8 +3,10 @@
 // Visit order and acceptance follow ClosestHit / AnyHit
 // (shaders/common/closest_hit.glsl:69-201, anyhit.glsl:48-170):
 //   * start at the top-level node without testing its box;
-//   * at an internal node test both child boxes, descend into the nearer one
-//     (ties to the left child) and push the farther one;
+//   * at an internal node test the child boxes, descend into the nearest one
+//     and push the others (the nodes are a BVH4 collapsed from the
+//     reference's binary tree with the same boxes, so the same triangles are
+//     reached);
 //   * at a TLAS leaf move the ray into instance space with the inverse
 //     transform, push a return marker and continue at the BLAS root;
 //   * at a BLAS leaf run Moller-Trumbore on each triangle.
@@ -16,39 +18,55 @@
 //     instead of the first one found, so the answer does not depend on visit
 //     order;
 //   * the per-lane stack keeps its newest entries in LDS (kLdsStack of them)
-//     and spills older ones to a per-lane global area.
+//     with their box entry distances, and spills older ones to a per-lane
+//     global area; a popped entry whose entry distance is beyond the current
+//     best t (times `cull`) is skipped.
 #pragma once
 #include "dev_common.h"
 
 namespace lfhd {
 
-constexpr int kLdsStack = 16;          // LDS entries per lane (power of 2)
-constexpr int kTraceBlock = 256;       // threads per block (4 wave64)
-constexpr uint32_t kSpillStride = 96;  // max stack depth per lane
 
 struct LaneStack {
-    uint32_t* lds;    // this lane's column: entry e at lds[e * 64]
-    uint32_t* spill;  // this lane's global spill area
+    uint32_t* lds;    // this lane's column of refs: entry e at lds[e * 64]
+    float* ldk;       // entry distances of the LDS entries, same indexing
+    uint32_t* spill;  // this lane's global spill area (refs only)
     int sp, lo;
     bool overflow;
     LFH_DEV void reset() { sp = 0; lo = 0; overflow = false; }
-    LFH_DEV void push(uint32_t v) {
+    LFH_DEV void push(uint32_t v, float key) {
         if (sp - lo == kLdsStack) {
             spill[lo] = lds[(lo & (kLdsStack - 1)) * 64];
             lo++;
         }
         if (sp >= (int)kSpillStride) { overflow = true; return; }
         lds[(sp & (kLdsStack - 1)) * 64] = v;
+        if (LFH_POP_CULL) ldk[(sp & (kLdsStack - 1)) * 64] = key;
         sp++;
     }
-    LFH_DEV uint32_t pop() {
-        if (sp <= 0) return kRefSentinel;
+    // Pops the newest entry; entries that come back from the spill area have
+    // lost their distance and are never culled.
+    LFH_DEV uint32_t pop(float& key) {
+        if (sp <= 0) {
+            key = -kInfinity;
+            return kRefSentinel;
+        }
         sp--;
         if (sp < lo) {
             lo = sp;
+            key = -kInfinity;
             return spill[sp];
         }
+        key = LFH_POP_CULL ? ldk[(sp & (kLdsStack - 1)) * 64] : -kInfinity;
         return lds[(sp & (kLdsStack - 1)) * 64];
+    }
+    // Pops until an entry whose box entry distance is within lim (or a marker).
+    LFH_DEV uint32_t pop_live(float lim) {
+        for (;;) {
+            float k;
+            const uint32_t r = pop(k);
+            if (r == kRefSentinel || k <= lim) return r;
+        }
     }
 };
 
@@ -105,7 +123,7 @@ LFH_DEV bool trace_run(const TraceScene& sc, TraceState& s, LaneStack& st, float
                 s.o = s.O;
                 s.d = s.D;
                 s.inv = s.invD;
-                s.cur = st.pop();
+                s.cur = st.pop_live(s.tbest * cull);
                 continue;
             }
             return true;
@@ -139,7 +157,7 @@ LFH_DEV bool trace_run(const TraceScene& sc, TraceState& s, LaneStack& st, float
                     }
                 }
             }
-            s.cur = st.pop();
+            s.cur = st.pop_live(s.tbest * cull);
         } else if (ref_is_inst(cur)) {
             const uint32_t inst = cur & 0x3FFFFFFFu;
             const float4* ip = sc.insts + (size_t)inst * 4;
@@ -148,36 +166,48 @@ LFH_DEV bool trace_run(const TraceScene& sc, TraceState& s, LaneStack& st, float
             s.o = xform_inv(r0, r1, r2, s.O, 1.0f);
             s.d = xform_inv(r0, r1, r2, s.D, 0.0f);
             s.inv = rcp3(s.d);
-            st.push(kRefSentinel);
+            st.push(kRefSentinel, -kInfinity);
             s.inBlas = true;
             s.curInst = (int)inst;
             s.curMat = (int)as_u(meta.y);
             s.cur = as_u(meta.x);
         } else {
-            const float4* np = sc.nodes + (size_t)cur * 4;
-            float4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3];
+            // BVH4 node: slab-test the four child boxes, descend into the
+            // nearest hit child, push the others farthest first
+            const float4* np = sc.nodes + (size_t)cur * 8;
+            const float4 mnx = np[0], mny = np[1], mnz = np[2], mxx = np[3], mxy = np[4], mxz = np[5];
+            const float4 rf = np[6];
             if (STATS) stats->nint++;
-            float el, er;
-            float hl = slab(v3(n0.x, n0.y, n0.z), v3(n0.w, n1.x, n1.y), s.o, s.inv, &el);
-            float hr = slab(v3(n1.z, n1.w, n2.x), v3(n2.y, n2.z, n2.w), s.o, s.inv, &er);
             const float lim = s.tbest * cull;
-            bool vl = hl > 0.f && el <= lim;
-            bool vr = hr > 0.f && er <= lim;
-            uint32_t L = as_u(n3.x), R = as_u(n3.y);
-            if (vl && vr) {
-                if (hl > hr) {
-                    s.cur = R;
-                    st.push(L);
-                } else {
-                    s.cur = L;
-                    st.push(R);
-                }
-            } else if (vl) {
-                s.cur = L;
-            } else if (vr) {
-                s.cur = R;
+            float h[4], e[4];
+            uint32_t r[4] = {as_u(rf.x), as_u(rf.y), as_u(rf.z), as_u(rf.w)};
+            h[0] = slab(v3(mnx.x, mny.x, mnz.x), v3(mxx.x, mxy.x, mxz.x), s.o, s.inv, &e[0]);
+            h[1] = slab(v3(mnx.y, mny.y, mnz.y), v3(mxx.y, mxy.y, mxz.y), s.o, s.inv, &e[1]);
+            h[2] = slab(v3(mnx.z, mny.z, mnz.z), v3(mxx.z, mxy.z, mxz.z), s.o, s.inv, &e[2]);
+            h[3] = slab(v3(mnx.w, mny.w, mnz.w), v3(mxx.w, mxy.w, mxz.w), s.o, s.inv, &e[3]);
+            int nv = 0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const bool ok = r[k] != kRefSentinel && h[k] > 0.f && e[k] <= lim;
+                nv += ok ? 1 : 0;
+                h[k] = ok ? fminf(h[k], 3.0e38f) : __builtin_inff();
+            }
+            // sort by hit distance (absent / culled children last)
+#define LFH_CSWAP(a, b)                                              \
+    if (h[b] < h[a]) {                                               \
+        float th = h[a]; h[a] = h[b]; h[b] = th;                     \
+        float te = e[a]; e[a] = e[b]; e[b] = te;                     \
+        uint32_t tr = r[a]; r[a] = r[b]; r[b] = tr;                  \
+    }
+            LFH_CSWAP(0, 1) LFH_CSWAP(2, 3) LFH_CSWAP(0, 2) LFH_CSWAP(1, 3) LFH_CSWAP(1, 2)
+#undef LFH_CSWAP
+            if (nv == 0) {
+                s.cur = st.pop_live(lim);
             } else {
-                s.cur = st.pop();
+                if (nv > 3) st.push(r[3], e[3]);
+                if (nv > 2) st.push(r[2], e[2]);
+                if (nv > 1) st.push(r[1], e[1]);
+                s.cur = r[0];
             }
         }
         if (st.overflow) return true;

@@ -23,7 +23,10 @@ using lfhd::kGuideParams;
 namespace {
 constexpr int kBatch = 65536;             // samples per optimisation step
 constexpr int kMaxSamples = 4 * kBatch;   // samples kept from one training pass
-constexpr int kRecordedPaths = 32768;     // target number of recorded paths per pass
+constexpr int kRecordedPaths = 1 << 20;   // target number of recorded paths per pass
+constexpr int kWarmSteps = 256;           // Adam steps taken at the warm-up rate
+constexpr int kWarmRepeats = 4;           // optimisation passes over a pass's samples while warming up
+constexpr float kWeightCap = 32.0f;       // online training: weights / their mean, capped
 }  // namespace
 
 struct lfh_guide {
@@ -212,18 +215,24 @@ int guide_pass_end(lfh_ctx* c, const GuideHook& gh) {
     hipError_t e = lfhd::guide_collect(b.rec.as<float4>(), b.nslots, c->opt.max_depth, gh.stride, gh.phase,
                                        c->pass.rad.as<float4>(), c->rs.npaths, b.block_counts.as<int>(),
                                        b.block_offsets.as<int>(), b.total.as<int>(), b.samples.as<float4>(),
-                                       kMaxSamples, c->stream);
+                                       kMaxSamples, b.block_wsum.as<float>(), b.total.as<float>() + 1, c->stream);
     int total = 0;
     if (e == hipSuccess) e = hipMemcpyAsync(&total, b.total.p, sizeof(int), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return ghip(g, e, "guide: collect");
     c->guide_samples += total;
-    for (int start = 0; start < total; start += kBatch) {
-        lfhd::GuideAdam ad = g->adam(true);
-        e = lfhd::guide_train(g->dev(), b.samples.as<float4>(), b.total.as<int>(), start, kBatch,
-                              b.partial.as<float>(), ad, nullptr, nullptr, c->stream);
-        if (e != hipSuccess) return ghip(g, e, "guide: train");
-        g->steps++;
+    // one Adam step per batch of the pass's samples; while the guide is young,
+    // several passes over them
+    const int reps = g->steps < kWarmSteps ? kWarmRepeats : 1;
+    for (int r = 0; r < reps && total > 0; ++r) {
+        for (int start = 0; start < total; start += kBatch) {
+            lfhd::GuideAdam ad = g->adam(true);
+            e = lfhd::guide_train(g->dev(), b.samples.as<float4>(), b.total.as<int>(), start, kBatch,
+                                  b.partial.as<float>(), ad, nullptr, nullptr, c->stream, b.total.as<float>() + 1,
+                                  kWeightCap);
+            if (e != hipSuccess) return ghip(g, e, "guide: train");
+            g->steps++;
+        }
     }
     return LFH_OK;
 }
@@ -396,6 +405,7 @@ int lfh_set_guide(lfh_ctx* c, lfh_guide* g, int32_t train_every) {
     if (e == hipSuccess) e = ensure(b.samples, (size_t)kMaxSamples * 4 * sizeof(float4));
     if (e == hipSuccess) e = ensure(b.block_counts, (size_t)(nblocks + 1) * sizeof(int));
     if (e == hipSuccess) e = ensure(b.block_offsets, (size_t)(nblocks + 1) * sizeof(int));
+    if (e == hipSuccess) e = ensure(b.block_wsum, (size_t)(nblocks + 1) * sizeof(float));
     if (e == hipSuccess) e = ensure(b.total, 64);
     if (e == hipSuccess) e = ensure(b.partial, lfhd::guide_partial_floats(kBatch) * sizeof(float));
     if (e != hipSuccess) {

@@ -36,8 +36,11 @@ LFH_DEV f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) { return __builtin_amdgcn_mfma_f
 LFH_DEV void encode_row(uint16_t* row, const GuideDev& g, V3 p, V3 n, V3 wo) {
     const V3 x = v3(guide_norm_coord(p.x, g.lo[0], g.inv_ext[0]), guide_norm_coord(p.y, g.lo[1], g.inv_ext[1]),
                     guide_norm_coord(p.z, g.lo[2], g.inv_ext[2]));
-#pragma unroll 4
-    for (int f = 0; f < kGuideIn; ++f) row[f] = to_bf16(guide_feature(f, x, n, wo));
+    float f[kGuideIn];
+    guide_encode(x, n, wo, f);
+#pragma unroll
+    for (int k = 0; k < kGuideIn; k += 2)
+        *reinterpret_cast<uint32_t*>(row + k) = (uint32_t)to_bf16(f[k]) | ((uint32_t)to_bf16(f[k + 1]) << 16);
 }
 
 // Forward pass of the 64 rows of X owned by this wave (rows r0 .. r0+63 in
@@ -215,6 +218,8 @@ struct TrainArgs {
     int start;             // first sample of this batch
     int batch;             // batch capacity (blocks x 64)
     float* partial;        // [blocks][kPartStride]
+    const float* wsum;     // device: sum of the sample weights (NULL: use the weights as given)
+    float wcap;            // with wsum: weights are w / mean(w), capped at wcap
 };
 
 LFH_DEV int batch_size(const TrainArgs& t) {
@@ -240,30 +245,28 @@ __global__ __launch_bounds__(kTrainBlock) void k_guide_train(GuideDev g, TrainAr
     float* part = ta.partial + (size_t)blockIdx.x * kPartStride;
     if (base >= N) return;  // block-uniform; the reduction reads only active blocks
 
-    // 1. encode: thread (s = tid & 63, quarter = tid >> 6) writes 16 features of sample s
-    {
-        const int s = tid & 63, qtr = tid >> 6;
+    // 1. encode: one thread per sample (rows past the batch are zero with w = 0)
+    if (tid < kTrainRows) {
+        const int s = tid;
         const bool ok = base + s < N;
-        V3 p = v3(0.f, 0.f, 0.f), nn = v3(0.f, 0.f, 1.f), wo = v3(0.f, 0.f, 1.f);
-        float w = 0.0f;
+        uint16_t* row = sA[0] + s * kPitch;
         if (ok) {
             const float4* sp = ta.samples + (size_t)(ta.start + base + s) * 4;
             const float4 a = sp[0], b = sp[1], c = sp[2], d = sp[3];
-            p = ld3(a);
-            w = a.w;
-            nn = ld3(b);
-            wo = ld3(c);
-            if (qtr == 0) {
-                sD[s][0] = d.x;
-                sD[s][1] = d.y;
-                sD[s][2] = d.z;
+            sD[s][0] = d.x;
+            sD[s][1] = d.y;
+            sD[s][2] = d.z;
+            float w = a.w;
+            if (ta.wsum) {
+                const float mean = *ta.wsum / (float)*ta.count;
+                w = mean > 0.0f ? fminf(w / mean, ta.wcap) : 0.0f;
             }
+            sD[s][3] = w;
+            encode_row(row, g, ld3(a), ld3(b), ld3(c));
+        } else {
+            sD[s][0] = sD[s][1] = sD[s][2] = sD[s][3] = 0.0f;
+            for (int f = 0; f < kGuideIn; ++f) row[f] = 0;
         }
-        if (qtr == 0) sD[s][3] = ok ? w : 0.0f;
-        const V3 x = v3(guide_norm_coord(p.x, g.lo[0], g.inv_ext[0]), guide_norm_coord(p.y, g.lo[1], g.inv_ext[1]),
-                        guide_norm_coord(p.z, g.lo[2], g.inv_ext[2]));
-        uint16_t* row = sA[0] + s * kPitch;
-        for (int f = qtr * 16; f < qtr * 16 + 16; ++f) row[f] = ok ? to_bf16(guide_feature(f, x, nn, wo)) : (uint16_t)0;
     }
     __syncthreads();
 
@@ -494,9 +497,11 @@ struct CollectArgs {
     int* total;
     float4* samples;
     int max_samples;
+    float* block_wsum; // per block: sum of the weights of its samples
+    float* wsum;       // sum of the weights of the kept samples
 };
 
-LFH_DEV bool collect_one(const CollectArgs& c, int slot, float4* out) {
+LFH_DEV bool collect_one(const CollectArgs& c, int slot, float4* out, float* wout = nullptr) {
     if (slot >= c.nslots) return false;
     const float4* r = c.rec + (size_t)slot * 4;
     const float4 r0 = r[0];
@@ -509,6 +514,7 @@ LFH_DEV bool collect_one(const CollectArgs& c, int slot, float4* out) {
     const float lf = 0.2126f * f.x + 0.7152f * f.y + 0.0722f * f.z;
     const float w = (lf - r2.w) / r1.w;
     if (!(w > 0.0f) || !isfinite(w)) return false;
+    if (wout) *wout = w;
     if (out) {
         const float4 r3 = r[3];
         out[0] = make_float4(r0.x, r0.y, r0.z, w);
@@ -520,15 +526,28 @@ LFH_DEV bool collect_one(const CollectArgs& c, int slot, float4* out) {
 }
 
 __global__ __launch_bounds__(256) void k_guide_count(CollectArgs c) {
+    __shared__ float sw[256];
     const int slot = blockIdx.x * 256 + threadIdx.x;
-    const int ok = collect_one(c, slot, nullptr) ? 1 : 0;
+    float w = 0.0f;
+    const int ok = collect_one(c, slot, nullptr, &w) ? 1 : 0;
+    sw[threadIdx.x] = ok ? w : 0.0f;
     const int cnt = __syncthreads_count(ok);
-    if (threadIdx.x == 0) c.block_counts[blockIdx.x] = cnt;
+    for (int off = 128; off > 0; off >>= 1) {   // fixed-order tree sum
+        if (threadIdx.x < off) sw[threadIdx.x] += sw[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        c.block_counts[blockIdx.x] = cnt;
+        c.block_wsum[blockIdx.x] = sw[0];
+    }
 }
 
-// Exclusive scan of the block counts (one block of 1024 threads).
+// Exclusive scan of the block counts (one block of 1024 threads) and the
+// fixed-order sum of the block weight sums over the blocks whose samples are
+// all kept.
 __global__ __launch_bounds__(1024) void k_guide_scan(CollectArgs c, int nblocks) {
     __shared__ int s[1024];
+    __shared__ float sws[1024];
     const int t = threadIdx.x;
     const int per = (nblocks + 1023) / 1024;
     const int b0 = t * per;
@@ -543,11 +562,20 @@ __global__ __launch_bounds__(1024) void k_guide_scan(CollectArgs c, int nblocks)
         __syncthreads();
     }
     int run = s[t] - local;
+    float wl = 0.0f;
     for (int b = b0; b < b0 + per && b < nblocks; ++b) {
         c.block_offsets[b] = run;
+        if (run + c.block_counts[b] <= c.max_samples) wl += c.block_wsum[b];
         run += c.block_counts[b];
     }
+    sws[t] = wl;
+    __syncthreads();
+    for (int off = 512; off > 0; off >>= 1) {
+        if (t < off) sws[t] += sws[t + off];
+        __syncthreads();
+    }
     if (t == 1023) *c.total = s[1023] < c.max_samples ? s[1023] : c.max_samples;
+    if (t == 0) *c.wsum = sws[0];
 }
 
 __global__ __launch_bounds__(256) void k_guide_compact(CollectArgs c) {
@@ -602,8 +630,9 @@ int guide_train_blocks(int batch) { return nblk(batch, kTrainRows); }
 size_t guide_partial_floats(int batch) { return (size_t)guide_train_blocks(batch) * kPartStride; }
 
 hipError_t guide_train(const GuideDev& g, const float4* samples, const int* count, int start, int batch,
-                       float* partial, const GuideAdam& adam, float* grad_out, float* loss_out, hipStream_t s) {
-    TrainArgs ta{samples, count, start, batch, partial};
+                       float* partial, const GuideAdam& adam, float* grad_out, float* loss_out, hipStream_t s,
+                       const float* wsum, float wcap) {
+    TrainArgs ta{samples, count, start, batch, partial, wsum, wcap};
     hipLaunchKernelGGL(k_guide_train, dim3(guide_train_blocks(batch)), dim3(kTrainBlock), 0, s, g, ta);
     AdamArgs ad{adam.master, adam.m, adam.v, adam.wf, adam.wb, adam.bias, adam.lr, adam.b1, adam.b2, adam.eps,
                 adam.c1, adam.c2};
@@ -619,9 +648,9 @@ hipError_t guide_refresh(const float* master, uint16_t* wf, uint16_t* wb, float*
 
 hipError_t guide_collect(const float4* rec, int nslots, int max_depth, int stride, int phase, const float4* rad,
                          int npaths, int* block_counts, int* block_offsets, int* total, float4* samples,
-                         int max_samples, hipStream_t s) {
+                         int max_samples, float* block_wsum, float* wsum, hipStream_t s) {
     CollectArgs c{rec, nslots, max_depth, stride, phase, rad, npaths, block_counts, block_offsets, total, samples,
-                  max_samples};
+                  max_samples, block_wsum, wsum};
     const int nb = nblk(nslots, 256);
     if (nb <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_guide_count, dim3(nb), dim3(256), 0, s, c);

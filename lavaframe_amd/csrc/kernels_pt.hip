@@ -77,6 +77,11 @@ __global__ __launch_bounds__(256) void k_generate(CamParams cam1, const JobDev* 
         pb.job[p] = j;
     }
     const int P = cam.tile_w * cam.tile_h;
+    if (!MULTI && cam.batch > 1 && inrange) {
+        const int k = lp / P;   // sample k of the batch: the frame after k passes
+        lp -= k * P;
+        cam.frame += k;
+    }
     bool valid = false;
     int lx = 0, ly = 0;
     if (inrange && lp < P) {
@@ -182,11 +187,13 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_queue(ShadeScene sc, cons
                                                              unsigned long long* __restrict__ work, float cull,
                                                              int* __restrict__ err) {
     __shared__ uint32_t stk[kTraceBlock / 64][kLdsStack][64];
+    __shared__ float stkd[kTraceBlock / 64][LFH_POP_CULL ? kLdsStack : 1][64];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const size_t gtid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     LaneStack st;
     st.lds = &stk[wave][0][lane];
+    st.ldk = &stkd[wave][0][lane];
     st.spill = spill + gtid * kSpillStride;
     st.reset();
     const int64_t n = (int64_t)*count;
@@ -371,8 +378,9 @@ __global__ __launch_bounds__(256) void k_shade(ShadeScene sc1, const JobDev* __r
             lastMat = matId;
             thr = thr * v3(expf(-absorb.x * t), expf(-absorb.y * t), expf(-absorb.z * t));
             uint4 rng = pb.rng[p];
-            const bool guided = gh.enabled != 0;
             const float4* lob = gh.lobes + (size_t)i * kGuideLobeF4;
+            // the strategy choice depends only on the lobes, never on a sample
+            const bool guided = gh.enabled != 0 && guide_valid(lob);
             // next-event estimation (pathtrace.glsl:126-204)
             const V3 V = -rd;
             V3 surfacePos = fhp + nrm * kEps;
@@ -522,23 +530,30 @@ __global__ __launch_bounds__(256) void k_accumulate(PathBufs pb, int total, CamP
     const int lx = lp % cam.tile_w, ly = lp / cam.tile_w;
     const int gx = cam.tileX * cam.tile_w + lx, gy = cam.tileY * cam.tile_h + ly;
     if (gx >= cam.width || gy >= cam.height) return;
-    V3 rad = ld3(pb.rad[p]);
-    combine_nee(pb, p, rad);
-    pb.rad[p] = f4(rad, 0.f);   // final path radiance (guide training targets)
     const size_t pix = (size_t)gy * cam.width + gx;
     float4 s = accum[pix];
-    if (kahan) {
-        float4 c = comp[pix];
-        float yx = rad.x - c.x, yy = rad.y - c.y, yz = rad.z - c.z;
-        float tx = s.x + yx, ty = s.y + yy, tz = s.z + yz;
-        c.x = (tx - s.x) - yx;
-        c.y = (ty - s.y) - yy;
-        c.z = (tz - s.z) - yz;
-        comp[pix] = c;
-        accum[pix] = make_float4(tx, ty, tz, s.w + 1.0f);
-    } else {
-        accum[pix] = make_float4(s.x + rad.x, s.y + rad.y, s.z + rad.z, s.w + 1.0f);
+    float4 c = kahan ? comp[pix] : make_float4(0.f, 0.f, 0.f, 0.f);
+    // the batch's samples of this pixel, added in frame order as separate passes would
+    const int nb = MULTI ? 1 : cam.batch;
+    const int P = cam.tile_w * cam.tile_h;
+    for (int k = 0; k < nb; ++k) {
+        const int q = p + k * P;
+        V3 rad = ld3(pb.rad[q]);
+        combine_nee(pb, q, rad);
+        pb.rad[q] = f4(rad, 0.f);   // final path radiance (guide training targets)
+        if (kahan) {
+            float yx = rad.x - c.x, yy = rad.y - c.y, yz = rad.z - c.z;
+            float tx = s.x + yx, ty = s.y + yy, tz = s.z + yz;
+            c.x = (tx - s.x) - yx;
+            c.y = (ty - s.y) - yy;
+            c.z = (tz - s.z) - yz;
+            s = make_float4(tx, ty, tz, s.w + 1.0f);
+        } else {
+            s = make_float4(s.x + rad.x, s.y + rad.y, s.z + rad.z, s.w + 1.0f);
+        }
     }
+    if (kahan) comp[pix] = c;
+    accum[pix] = s;
 }
 
 // radiance out for lfh_path_trace_rays
@@ -668,7 +683,7 @@ hipError_t pt_reset(uint32_t* counters, unsigned long long* work, hipStream_t s)
 }
 
 hipError_t pt_generate(const CamParams& cam, const PathBufs& pb, float4* rayq, uint32_t* count, hipStream_t s) {
-    int P = cam.tile_w * cam.tile_h;
+    int P = cam.tile_w * cam.tile_h * (cam.batch > 1 ? cam.batch : 1);
     hipLaunchKernelGGL(k_generate<false>, dim3(nblk(P)), dim3(256), 0, s, cam, (const JobDev*)nullptr, 1, P, pb, rayq,
                        count);
     return hipGetLastError();

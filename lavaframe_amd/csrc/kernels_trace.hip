@@ -22,11 +22,13 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceScene sc, const floa
                                                        unsigned long long* __restrict__ work, float cull,
                                                        int* __restrict__ err) {
     __shared__ uint32_t stk[kTraceBlock / 64][kLdsStack][64];
+    __shared__ float stkd[kTraceBlock / 64][LFH_POP_CULL ? kLdsStack : 1][64];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const size_t gtid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     LaneStack st;
     st.lds = &stk[wave][0][lane];
+    st.ldk = &stkd[wave][0][lane];
     st.spill = spill + gtid * kSpillStride;
     st.reset();
 

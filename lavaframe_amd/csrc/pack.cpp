@@ -2,6 +2,7 @@
 // Renderer.cpp:76-188, re-designed for gfx950 loads).
 #include <cmath>
 #include <cstring>
+#include <unordered_map>
 #include <string>
 #include <vector>
 
@@ -251,10 +252,146 @@ struct Builder {
 };
 }  // namespace
 
+namespace {
+// Collapses the packed BVH2 into a BVH4: each BVH4 node takes the two children
+// of a BVH2 node and keeps replacing its internal child of largest surface
+// area by that child's two children until it has four.  Every child keeps the
+// exact float box of its BVH2 node, and a BVH2 box contains its children's
+// boxes, so a ray passes a collapsed child's slab test only if it passes the
+// tests of the BVH2 nodes skipped above it: the triangles tested (and the
+// hits found) are those of the binary traversal.  Nodes are allocated in
+// preorder, so a node's first child follows it in memory.
+struct Widen {
+    const std::vector<float>& n2;
+    std::vector<float>& out;
+    std::unordered_map<uint32_t, std::pair<uint32_t, int>> memo;  // BLAS root -> (ref4, need)
+    const std::vector<float>& insts;
+
+    struct Child {
+        float box[6];
+        uint32_t ref;
+    };
+    static float area(const float* b) {
+        float dx = b[3] - b[0], dy = b[4] - b[1], dz = b[5] - b[2];
+        return dx * dy + dy * dz + dz * dx;
+    }
+    void children_of(uint32_t r, Child* c) const {
+        const float* o = &n2[(size_t)r * 16];
+        std::memcpy(c[0].box, o, 6 * sizeof(float));
+        std::memcpy(c[1].box, o + 6, 6 * sizeof(float));
+        std::memcpy(&c[0].ref, o + 12, 4);
+        std::memcpy(&c[1].ref, o + 13, 4);
+    }
+    static bool internal(uint32_t r) { return r != lfhd::kRefSentinel && !lfhd::ref_is_leaf(r) && !lfhd::ref_is_inst(r); }
+
+    // Returns the BVH4 ref of BVH2 ref r and, in *need, the largest number of
+    // stack entries a traversal below it can hold.
+    uint32_t build(uint32_t r, int* need) {
+        if (lfhd::ref_is_inst(r)) {
+            uint32_t inst = r & 0x3FFFFFFFu, blas;
+            std::memcpy(&blas, &insts[(size_t)inst * 16 + 12], 4);
+            *need = 1 + blas_ref(blas).second;  // the BLAS-return marker, then the BLAS
+            return r;
+        }
+        if (!internal(r)) {
+            *need = 0;
+            return r;
+        }
+        Child ch[4];
+        children_of(r, ch);
+        int n = 2;
+        while (n < 4) {
+            int best = -1;
+            float ba = -1.0f;
+            for (int k = 0; k < n; ++k)
+                if (internal(ch[k].ref) && area(ch[k].box) > ba) {
+                    ba = area(ch[k].box);
+                    best = k;
+                }
+            if (best < 0) break;
+            Child sub[2];
+            children_of(ch[best].ref, sub);
+            for (int k = n; k > best + 1; --k) ch[k] = ch[k - 1];
+            ch[best] = sub[0];
+            ch[best + 1] = sub[1];
+            ++n;
+        }
+        const size_t idx = out.size() / 32;
+        out.resize(out.size() + 32, 0.0f);
+        uint32_t refs[4] = {lfhd::kRefSentinel, lfhd::kRefSentinel, lfhd::kRefSentinel, lfhd::kRefSentinel};
+        int worst = 0;
+        for (int k = 0; k < n; ++k) {
+            int nk = 0;
+            refs[k] = build(ch[k].ref, &nk);
+            worst = std::max(worst, nk);
+        }
+        float* o = &out[idx * 32];
+        for (int k = 0; k < 4; ++k) {
+            const float* b = k < n ? ch[k].box : nullptr;
+            for (int a = 0; a < 6; ++a) o[a * 4 + k] = b ? b[a] : (a < 3 ? 1.0f : -1.0f);
+            std::memcpy(&o[24 + k], &refs[k], 4);
+        }
+        *need = (n - 1) + worst;
+        return (uint32_t)idx;
+    }
+    std::pair<uint32_t, int> blas_ref(uint32_t r) {
+        auto it = memo.find(r);
+        if (it != memo.end()) return it->second;
+        int need = 0;
+        uint32_t r4 = build(r, &need);
+        memo[r] = {r4, need};
+        return {r4, need};
+    }
+};
+}  // namespace
+
+std::string widen_bvh(PackedScene* ps) {
+    ps->nodes4.clear();
+    Widen w{ps->nodes, ps->nodes4, {}, ps->insts};
+    // BLAS trees first, in instance order, then the TLAS
+    for (int i = 0; i < ps->num_instances; ++i) {
+        uint32_t blas;
+        std::memcpy(&blas, &ps->insts[(size_t)i * 16 + 12], 4);
+        w.blas_ref(blas);
+    }
+    const size_t blas4 = ps->nodes4.size() / 32;
+    int need = 0;
+    ps->root4 = w.build(ps->root, &need);
+    // The TLAS part gets room for one BVH4 node per TLAS BVH2 node (an upper
+    // bound for any boxes), so instance updates never change the array size.
+    size_t tlas2 = 0;
+    std::vector<uint32_t> st;
+    if (Widen::internal(ps->root)) st.push_back(ps->root);
+    while (!st.empty()) {
+        uint32_t r = st.back();
+        st.pop_back();
+        ++tlas2;
+        Widen::Child c[2];
+        w.children_of(r, c);
+        for (auto& k : c)
+            if (Widen::internal(k.ref)) st.push_back(k.ref);
+    }
+    const size_t cap = blas4 + std::max<size_t>(tlas2, 1);
+    if (ps->nodes4.size() / 32 < cap) ps->nodes4.resize(cap * 32, 0.0f);
+    if ((uint64_t)ps->nodes4.size() / 32 >= (1ull << 30)) return "too many BVH4 nodes";
+    ps->stack_need = need;
+    if (need > (int)lfhd::kSpillStride)
+        return "BVH too deep for the traversal stack (" + std::to_string(need) + " entries needed, " +
+               std::to_string(lfhd::kSpillStride) + " available)";
+    ps->insts4 = ps->insts;
+    for (int i = 0; i < ps->num_instances; ++i) {
+        uint32_t blas;
+        std::memcpy(&blas, &ps->insts[(size_t)i * 16 + 12], 4);
+        uint32_t b4 = w.memo.count(blas) ? w.memo[blas].first : blas;
+        std::memcpy(&ps->insts4[(size_t)i * 16 + 12], &b4, 4);
+    }
+    return std::string();
+}
+
 std::string pack_scene(const lfh_scene_desc& d, PackedScene* out) {
     Builder b(d, out);
     if (!b.run()) return b.err.empty() ? std::string("invalid scene") : b.err;
-    return std::string();
+    return widen_bvh(out);
 }
 
 // Renderer::Update re-uploads transforms, materials and the TLAS node range
@@ -329,6 +466,10 @@ std::string repack_instances(PackedScene* ps, const float* transforms, const flo
             uint32_t m = (uint32_t)mat[i];
             std::memcpy(&ps->insts[(size_t)i * 16 + 13], &m, 4);
         }
+    const size_t n4 = ps->nodes4.size();
+    std::string err = widen_bvh(ps);
+    if (!err.empty()) return err;
+    if (ps->nodes4.size() != n4) return "TLAS update changes the node structure";
     return std::string();
 }
 
@@ -342,9 +483,9 @@ static hipError_t up(DevBuf& b, const std::vector<T>& v, hipStream_t s) {
 
 hipError_t upload_scene(const PackedScene& ps, DeviceScene* ds, hipStream_t s) {
     hipError_t e;
-    if ((e = up(ds->nodes, ps.nodes, s)) != hipSuccess) return e;
+    if ((e = up(ds->nodes, ps.nodes4, s)) != hipSuccess) return e;
     if ((e = up(ds->tris, ps.tris, s)) != hipSuccess) return e;
-    if ((e = up(ds->insts, ps.insts, s)) != hipSuccess) return e;
+    if ((e = up(ds->insts, ps.insts4, s)) != hipSuccess) return e;
     if ((e = up(ds->xforms, ps.xforms, s)) != hipSuccess) return e;
     if ((e = up(ds->nmats, ps.nmats, s)) != hipSuccess) return e;
     if ((e = up(ds->vidx, ps.vidx, s)) != hipSuccess) return e;
@@ -362,8 +503,8 @@ lfhd::TraceScene trace_view(const PackedScene& ps, const DeviceScene& ds) {
     t.nodes = ds.nodes.as<const float4>();
     t.tris = ds.tris.as<const float4>();
     t.insts = ds.insts.as<const float4>();
-    t.root = ps.root;
-    t.spill_stride = 96;
+    t.root = ps.root4;
+    t.spill_stride = lfhd::kSpillStride;
     return t;
 }
 

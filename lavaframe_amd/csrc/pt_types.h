@@ -42,6 +42,7 @@ struct CamParams {
     float fov, focal_dist, aperture;
     int width, height, tile_w, tile_h, tileX, tileY;
     int frame;
+    int batch;   // samples per pixel in this wavefront: path p is pixel p % P of frame + p / P
 };
 
 // One job of a multi-job (serving) pass: its scene, camera and frame, its

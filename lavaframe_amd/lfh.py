@@ -150,6 +150,7 @@ _SIGS = {
     "lfh_set_guide": (C.c_int, [_P, _P, C.c_int32]),
     "lfh_guide_steps": (C.c_int, [_P, i64p]),
     "lfh_get_ray_counts": (C.c_int, [_P, i64p, i64p]),
+    "lfh_set_frame_offset": (C.c_int, [_P, C.c_int32]),
     "lfh_guide_sample": (C.c_int, [_P, f32p, u32p, C.c_int64, f32p, f32p]),
     "lfh_guide_pdf": (C.c_int, [_P, f32p, f32p, C.c_int64, f32p]),
     "lfh_guide_samples_seen": (C.c_int, [_P, i64p]),
@@ -174,7 +175,7 @@ def load(path: str | os.PathLike | None = None):
         import torch  # noqa: F401
     except Exception:  # pragma: no cover - torch is part of the image
         pass
-    p = Path(path) if path else LIB_PATH
+    p = Path(path) if path else Path(os.environ.get("LFH_LIBRARY", LIB_PATH))
     if not p.exists():
         raise RuntimeError(f"lfh native library not built: {p} (run lavaframe_amd/build.py)")
     lib = C.CDLL(str(p))
@@ -556,6 +557,9 @@ class Context:
         pdf = np.zeros(d.shape[0], np.float32)
         self._chk(self.lib.lfh_env_pdf(self.h, fptr(d), d.shape[0], fptr(pdf)))
         return pdf
+
+    def set_frame_offset(self, offset: int):
+        self._chk(self.lib.lfh_set_frame_offset(self.h, int(offset)))
 
     def ray_counts(self):
         a, b = C.c_int64(), C.c_int64()

@@ -63,6 +63,7 @@ def lib():
         L.orc_render.argtypes = [P, C.POINTER(Options), C.c_int32, C.c_int32, f64p, C.c_int, C.c_int]
         L.orc_render_pixels.argtypes = [P, C.POINTER(Options), i32p, C.c_int64, C.c_int32, C.c_int32, f64p, f64p,
                                         C.c_int, C.c_int]
+        L.orc_camera_rays.argtypes = [C.POINTER(Options), C.c_int32, C.c_int32, C.c_int32, f64p, C.c_int]
         _lib = L
     return _lib
 
@@ -185,6 +186,13 @@ class OracleScene:
         self.L.orc_path_trace_rays(self.h, C.byref(opts), rp, sd.ctypes.data_as(u32p), n, rad.ctypes.data_as(f32p),
                                    precision, nthreads or threads())
         return rad
+
+    @staticmethod
+    def camera_rays(opts: Options, frame: int, tile=(0, 0), precision=64):
+        """(tile_h, tile_w, 6) origins and directions of one tile's camera rays."""
+        out = np.zeros((opts.tile_height, opts.tile_width, 6), np.float64)
+        lib().orc_camera_rays(C.byref(opts), tile[0], tile[1], frame, out.ctypes.data_as(f64p), precision)
+        return out
 
     def render(self, opts: Options, frame0: int, passes: int, precision=32, nthreads=None):
         acc = np.zeros((opts.height, opts.width, 3), np.float64)

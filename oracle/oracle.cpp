@@ -1279,6 +1279,26 @@ void orc_path_trace_rays(const orc_scene* s, const orc_options* o, const float* 
     }
 }
 
+void orc_camera_rays(const orc_options* o, int32_t tileX, int32_t tileY, int32_t frame, double* rays6, int precision) {
+    const int tw = o->tile_width, th = o->tile_height;
+    for (int ly = 0; ly < th; ++ly)
+        for (int lx = 0; lx < tw; ++lx) {
+            uint32_t seed[4];
+            double* r = rays6 + ((size_t)ly * tw + lx) * 6;
+            if (precision == 64) {
+                Ray<double> ray;
+                camera_ray<double>(o, tileX, tileY, lx, ly, frame, ray, seed);
+                r[0] = ray.origin.x, r[1] = ray.origin.y, r[2] = ray.origin.z;
+                r[3] = ray.direction.x, r[4] = ray.direction.y, r[5] = ray.direction.z;
+            } else {
+                Ray<float> ray;
+                camera_ray<float>(o, tileX, tileY, lx, ly, frame, ray, seed);
+                r[0] = ray.origin.x, r[1] = ray.origin.y, r[2] = ray.origin.z;
+                r[3] = ray.direction.x, r[4] = ray.direction.y, r[5] = ray.direction.z;
+            }
+        }
+}
+
 void orc_render(const orc_scene* s, const orc_options* o, int32_t frame0, int32_t passes, double* accum, int precision,
                 int threads) {
     if (threads > 0) omp_set_num_threads(threads);

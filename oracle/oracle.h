@@ -97,6 +97,9 @@ void orc_path_trace_rays(const orc_scene* s, const orc_options* o, const float* 
 /* One full pass of renderer.glsl:main over the frame (tile by tile, in the
  * TiledRenderer order) for frames [frame0, frame0 + passes): accum (w*h*3,
  * double) += each pass's samples.  Pixels are bottom-up rows like GL. */
+/* renderer.glsl:27-62 camera rays of one tile (origin, direction per tile
+ * pixel, row-major), for the given tile and frame. */
+void orc_camera_rays(const orc_options* o, int32_t tileX, int32_t tileY, int32_t frame, double* rays6, int precision);
 void orc_render(const orc_scene* s, const orc_options* o, int32_t frame0, int32_t passes, double* accum,
                 int precision, int threads);
 /* Same, for a subset of pixels only (pixel ids y*w + x), accumulating per

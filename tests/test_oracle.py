@@ -385,3 +385,158 @@ def test_fp32_and_fp64_paths_agree_in_the_mean():
     a32 = orc.render(o, 2, 64, precision=32) / 64
     a64 = orc.render(o, 2, 64, precision=64) / 64
     assert abs(a32.mean() - a64.mean()) < 0.02 * a64.mean()
+
+
+# ---------------------------------------------------------------- transmission, absorption, camera
+def _fresnel_exact(cos_i, n1, n2):
+    """Unpolarised Fresnel reflectance (textbook, Hecht's Optics 4.34/4.40)."""
+    sin_t = n1 / n2 * math.sqrt(max(0.0, 1 - cos_i * cos_i))
+    if sin_t >= 1:
+        return 1.0
+    cos_t = math.sqrt(1 - sin_t * sin_t)
+    rs = (n1 * cos_i - n2 * cos_t) / (n1 * cos_i + n2 * cos_t)
+    rp = (n2 * cos_i - n1 * cos_t) / (n2 * cos_i + n1 * cos_t)
+    return 0.5 * (rs * rs + rp * rp)
+
+
+@pytest.mark.parametrize("theta_deg,ior", [(30.0, 1.5), (60.0, 1.33), (10.0, 2.0)])
+def test_smooth_dielectric_sampling_follows_snell_and_fresnel(theta_deg, ior):
+    """specTrans = 1, metallic = 0, roughness at its floor: DisneySample reflects
+    with the exact Fresnel probability and refracts along Snell's law
+    (disney.glsl:128-173 against the textbook)."""
+    orc = _bsdf_scene([lfh.material((1, 1, 1), transmission=1.0, ior=ior, roughness=0.0)])
+    th = math.radians(theta_deg)
+    V = np.array([math.sin(th), 0.0, math.cos(th)])
+    N = 40000
+    fr = frame_rows(V.astype(np.float32), 1, eta=1.0 / ior, n=N)
+    seeds = np.random.default_rng(11).integers(0, 2**32, (N, 4), dtype=np.uint64).astype(np.uint32)
+    L, f, pdf, _ = orc.sample_bsdf(fr, seeds, 64)
+    refl = L[:, 2] > 0
+    F = _fresnel_exact(math.cos(th), 1.0, ior)
+    se = math.sqrt(F * (1 - F) / N)
+    assert abs(refl.mean() - F) < 5 * se + 1e-4, (refl.mean(), F)
+    # the roughness floor (GGX alpha 1e-3) spreads the microfacet normals with a
+    # long tail: the median deviation stays at the alpha scale and 95 % within 1e-2
+    def dev(Ls, target):
+        return np.arccos(np.clip(Ls @ target, -1, 1))
+    mirror = np.array([-V[0], -V[1], V[2]])
+    dm = dev(L[refl], mirror)
+    assert np.median(dm) < 3e-3 and np.quantile(dm, 0.95) < 1e-2, (np.median(dm), np.quantile(dm, 0.95))
+    # Snell: sin(theta_t) = sin(theta_i) / ior, in the plane of incidence, below the surface
+    st = math.sin(th) / ior
+    t_dir = np.array([-st, 0.0, -math.sqrt(1 - st * st)])
+    dt = dev(L[~refl], t_dir)
+    assert np.median(dt) < 3e-3 and np.quantile(dt, 0.95) < 1e-2, (np.median(dt), np.quantile(dt, 0.95))
+
+
+def test_beer_lambert_absorption_through_a_cube():
+    """A nearly index-matched medium (ior 1.01, roughness at its floor) with
+    extinction e and atDistance 1 inside the unit cube, lit by a constant white
+    background: a ray crossing the cube over length l (Snell's law inside)
+    arrives with e^l per channel (pathtrace.glsl:262-276: exp(-absorption t),
+    absorption = -log(e) / atDistance), less the two Fresnel reflections.
+    (ior exactly 1 makes the refraction term 0/0 in the reference formula.)"""
+    ext = (0.5, 0.25, 0.8)
+    ior = 1.01
+    s = lfh.Scene()
+    s.add_material(lfh.MATERIAL_DEFAULT)
+    s.add_material(lfh.material((1, 1, 1), transmission=1.0, ior=ior, roughness=0.0, extinction=ext, atDistance=1.0))
+    s.add_instance(s.add_mesh(scenes.unit_cube_tris()), None, 1)
+    s.set_camera((0, 0, -3), (0, 0, 0), 40)
+    s.set_options(width=8, height=8, max_depth=6, enable_rr=0, use_constant_bg=1, bg_color=(1.0, 1.0, 1.0))
+    sc = s.build()
+    orc = native.OracleScene(sc.arrays())
+    o = native.options_from(sc.options(), sc.camera(), lights=False, envmap=False)
+    rng = np.random.default_rng(2)
+    n = 256
+    for slope in (0.0, 0.25):
+        d = np.array([1.0, slope, 0.0]) / math.hypot(1.0, slope)
+        rays = np.zeros((n, 6), np.float32)
+        y0 = rng.uniform(-0.4, 0.5 - slope - 0.1, n)
+        z0 = rng.uniform(-0.4, 0.4, n)
+        rays[:, 0] = -2.0
+        rays[:, 1] = y0 - 1.5 * slope
+        rays[:, 2] = z0
+        rays[:, 3:6] = d
+        seeds = rng.integers(0, 2**32, (n, 4), dtype=np.uint64).astype(np.uint32)
+        rad = orc.path_trace_rays(o, rays, seeds, 64)
+        th = math.atan(slope)
+        tt = math.asin(math.sin(th) / ior)
+        length = 1.0 / math.cos(tt)
+        T = (1 - _fresnel_exact(math.cos(th), 1.0, ior)) * (1 - _fresnel_exact(math.cos(tt), ior, 1.0))
+        # the continuation ray starts EPS = 1e-3 along L past the entry point
+        # (pathtrace.glsl:291), so the distance absorbed over is l - EPS; a path
+        # that reflects (probability ~1e-4 per face) misses the expected value
+        expect = T * np.array(ext) ** (length - 1e-3)
+        ok = np.isclose(rad, expect, rtol=5e-3).all(1)
+        assert ok.mean() > 0.98, (ok.mean(), rad[~ok][:3], expect)
+        # the microfacet normals (GGX alpha 1e-3) jitter single paths; the median is sharp
+        np.testing.assert_allclose(np.median(rad[ok], 0), expect, rtol=2e-4)
+
+
+def _cam_opts(W, H, fov_deg, aperture=0.0, focal=1.0):
+    o = native.Options()
+    o.width, o.height, o.tile_width, o.tile_height = W, H, W, H
+    o.cam_pos[:] = [1.0, 2.0, 3.0]
+    o.cam_right[:] = [1.0, 0.0, 0.0]
+    o.cam_up[:] = [0.0, 1.0, 0.0]
+    o.cam_fwd[:] = [0.0, 0.0, 1.0]
+    o.fov = math.radians(fov_deg)
+    o.aperture = aperture
+    o.focal_dist = focal
+    return o
+
+
+def test_camera_pixel_mapping_and_tent_filter():
+    """Pinhole camera (renderer.glsl:27-62): origins at the camera, the
+    horizontal field of view maps pixel x + 0.5 + jitter to
+    tan(fov/2) (2u/W - 1), rows scaled by H/W; the jitter of each axis follows
+    the tent density 1 - |j| on [-1, 1] pixels."""
+    W, H, fov = 64, 48, 60.0
+    o = _cam_opts(W, H, fov)
+    tan = math.tan(math.radians(fov) / 2)
+    xs = np.arange(W) + 0.5
+    ys = np.arange(H) + 0.5
+    jx, jy = [], []
+    for frame in range(1, 201):
+        r = native.OracleScene.camera_rays(o, frame)
+        np.testing.assert_array_equal(r[..., 0:3], np.broadcast_to([1.0, 2.0, 3.0], r[..., 0:3].shape))
+        d = r[..., 3:6]
+        u = (d[..., 0] / d[..., 2] / tan + 1) * W / 2
+        v = (d[..., 1] / d[..., 2] / (tan * H / W) + 1) * H / 2
+        jx.append(u - xs[None, :])
+        jy.append(v - ys[:, None])
+    for j in (np.concatenate(jx).ravel(), np.concatenate(jy).ravel()):
+        assert np.abs(j).max() <= 1 + 1e-6
+        assert abs(j.mean()) < 5 * math.sqrt(1 / 6 / j.size)
+        edges = np.linspace(-1, 1, 21)
+        obs = np.histogram(j, edges)[0]
+        cdf = lambda x: np.where(x < 0, 0.5 * (1 + x) ** 2, 1 - 0.5 * (1 - x) ** 2)  # noqa: E731
+        exp = np.diff(cdf(edges)) * j.size
+        z = np.abs(obs - exp) / np.sqrt(exp)
+        assert z.max() < 6, z.max()
+
+
+def test_thin_lens_rays_meet_at_the_focal_point():
+    """Thin lens: every lens sample of a pixel passes through the point at
+    focalDist along the pinhole ray with the same jitter; lens positions fill
+    a disk of radius sqrt(aperture) uniformly (E|a|^2 = aperture / 2)."""
+    W, H = 32, 24
+    ap, focal = 0.04, 2.5
+    pin = _cam_opts(W, H, 50.0)
+    lens = _cam_opts(W, H, 50.0, aperture=ap, focal=focal)
+    pos = np.array([1.0, 2.0, 3.0])
+    r2 = []
+    for frame in range(1, 41):
+        p = native.OracleScene.camera_rays(pin, frame)
+        q = native.OracleScene.camera_rays(lens, frame)
+        P = pos + focal * p[..., 3:6]
+        off = q[..., 0:3] - pos
+        assert np.all(np.abs(off[..., 2]) < 1e-12)
+        r2.append((off ** 2).sum(-1))
+        w = P - q[..., 0:3]
+        dist = np.linalg.norm(w - (w * q[..., 3:6]).sum(-1, keepdims=True) * q[..., 3:6], axis=-1)
+        assert dist.max() < 1e-9
+    r2 = np.concatenate(r2).ravel()
+    assert r2.max() <= ap + 1e-12
+    assert abs(r2.mean() - ap / 2) < 5 * (ap / math.sqrt(12)) / math.sqrt(r2.size)

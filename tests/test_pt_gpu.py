@@ -277,3 +277,26 @@ def test_kahan_accumulation():
     exact = passes * float(np.float32(0.1))
     assert np.all(np.abs(out[1] - exact) <= 4 * np.spacing(np.float32(exact)))
     assert np.abs(out[0] - exact).max() > np.abs(out[1] - exact).max()
+
+
+def test_batched_passes_equal_sequential_passes():
+    """lfh_render_passes over a one-tile frame renders many passes in one
+    wavefront; the accumulation (sum and compensation), sample counter and
+    frame number equal those of the same passes rendered one by one."""
+    print("[pt] batched passes", flush=True)
+    sc = scenes.cornell(64, 64)
+    cam = sc.camera()
+    a = ctx_of(sc)
+    a.update(cam, reset=True)
+    a.render_passes(100)            # batches of 64 + 36
+    b = ctx_of(sc)
+    b.update(cam, reset=True)
+    for _ in range(100):
+        b.update(cam)
+        b.render()
+    assert (a.sample_count(), a.frame()) == (b.sample_count(), b.frame())
+    sa, ca = a.accum()
+    sb, cb = b.accum()
+    np.testing.assert_array_equal(sa, sb)
+    np.testing.assert_array_equal(ca, cb)
+    assert sa.sum() > 0
