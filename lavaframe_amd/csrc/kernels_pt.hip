@@ -383,7 +383,10 @@ __global__ __launch_bounds__(256) void k_shade(ShadeScene sc1, const JobDev* __r
             const bool guided = gh.enabled != 0 && guide_valid(lob);
             // next-event estimation (pathtrace.glsl:126-204)
             const V3 V = -rd;
-            V3 surfacePos = fhp + nrm * kEps;
+            // the reference offsets along the unflipped normal (pathtrace.glsl:130), so
+            // from behind a one-sided surface its shadow rays start on the far side;
+            // the consistent estimator offsets to the side the ray came from
+            V3 surfacePos = fhp + (sc.estimator == 1 ? fr.N : nrm) * kEps;
             shO = surfacePos;
             float4 e4 = make_float4(0.f, 0.f, 0.f, 0.f), l4 = make_float4(0.f, 0.f, 0.f, 0.f);
             if (sc.use_env) {

@@ -311,12 +311,21 @@ def aperture_room(width=1280, height=720, max_depth=10, target_tris=200000, seed
     tris += _quad(P(4, 0, 0), P(4, 0, z0), P(4, 3, z0), P(4, 3, 0))
     tris += _quad(P(4, 0, z1), P(4, 0, 4), P(4, 3, 4), P(4, 3, z1))
     tris += _quad(P(4, h, z0), P(4, h, z1), P(4, 3, z1), P(4, 3, z0))
-    # annex x in [4, 6]
-    tris += _quad(P(4, 0, 0), P(6, 0, 0), P(6, 0, 4), P(4, 0, 4))
-    tris += _quad(P(4, 3, 0), P(4, 3, 4), P(6, 3, 4), P(6, 3, 0))
-    tris += _quad(P(6, 0, 0), P(6, 3, 0), P(6, 3, 4), P(6, 0, 4))
-    tris += _quad(P(4, 0, 0), P(4, 3, 0), P(6, 3, 0), P(6, 0, 0))
-    tris += _quad(P(4, 0, 4), P(6, 0, 4), P(6, 3, 4), P(4, 3, 4))
+    # annex x in [4.02, 6], every face towards its inside; the dividing wall is
+    # 2 cm thick, its annex face at x = 4.02 with the same door, and jambs line
+    # the door opening
+    xa = 4.02
+    tris += _quad(P(xa, 0, 0), P(xa, 0, 4), P(6, 0, 4), P(6, 0, 0))          # annex floor (+y)
+    tris += _quad(P(xa, 3, 0), P(6, 3, 0), P(6, 3, 4), P(xa, 3, 4))          # annex ceiling (-y)
+    tris += _quad(P(6, 0, 0), P(6, 0, 4), P(6, 3, 4), P(6, 3, 0))            # x = 6 (-x)
+    tris += _quad(P(xa, 0, 0), P(6, 0, 0), P(6, 3, 0), P(xa, 3, 0))          # z = 0 (+z)
+    tris += _quad(P(xa, 0, 4), P(xa, 3, 4), P(6, 3, 4), P(6, 0, 4))          # z = 4 (-z)
+    tris += _quad(P(xa, 0, 0), P(xa, 3, 0), P(xa, 3, z0), P(xa, 0, z0))      # annex face of the wall (+x)
+    tris += _quad(P(xa, 0, z1), P(xa, 3, z1), P(xa, 3, 4), P(xa, 0, 4))
+    tris += _quad(P(xa, h, z0), P(xa, 3, z0), P(xa, 3, z1), P(xa, h, z1))
+    tris += _quad(P(4, 0, z0), P(xa, 0, z0), P(xa, h, z0), P(4, h, z0))      # jamb z = z0 (+z)
+    tris += _quad(P(4, 0, z1), P(4, h, z1), P(xa, h, z1), P(xa, 0, z1))      # jamb z = z1 (-z)
+    tris += _quad(P(4, h, z0), P(xa, h, z0), P(xa, h, z1), P(4, h, z1))      # lintel (-y)
     room = s.add_mesh(np.array(tris, np.float32))
     s.add_instance(room, affine(), wall)
     per = max(1000, target_tris // 8)

@@ -887,7 +887,9 @@ template <class T> void GetMaterial(const orc_scene* s, State<T>& st, const Ray<
 template <class T>
 V3<T> DirectLight(const orc_scene* s, const orc_options* o, const Ray<T>& r, const State<T>& st, uint32_t* seed) {
     V3<T> Li = mk(T(0), T(0), T(0));
-    V3<T> surfacePos = st.fhp + st.normal * C<T>::EPS;
+    // estimator 1: offset to the side the ray came from (the reference uses the
+    // unflipped normal, pathtrace.glsl:130)
+    V3<T> surfacePos = st.fhp + (o->estimator == 1 ? st.ffnormal : st.normal) * C<T>::EPS;
     BsdfSampleRec<T> b;
     if (o->envmap && !o->constant_bg) {
         V3<T> color, lightDir;

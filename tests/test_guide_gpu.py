@@ -236,3 +236,54 @@ def test_guided_render_is_unbiased():
     z = np.abs(bm(img) - bm(mean)) / np.maximum(se, 1e-6)
     assert np.mean(z < 4.5) > 0.99 and z.max() < 7, (np.mean(z < 4.5), z.max())
     assert abs(img.mean() - mean.mean()) < 0.02 * mean.mean()
+
+
+def _floor_scene(flipped):
+    """A 4 x 4 floor (y = 0) under a quad light facing down; flipped: the
+    floor's winding makes its normal point down, away from the light."""
+    s = lfh.Scene()
+    s.add_material(lfh.MATERIAL_DEFAULT)
+    s.add_material(lfh.material((0.7, 0.7, 0.7), roughness=0.6))
+    a, b, c, d = [np.array(p, np.float32) for p in ((-2, 0, -2), (-2, 0, 2), (2, 0, 2), (2, 0, -2))]
+    tris = np.array([[a, b, c], [a, c, d]], np.float32)
+    if flipped:
+        tris = tris[:, ::-1].copy()
+    s.add_instance(s.add_mesh(tris), None, 1)
+    s.add_light(lfh.quad_light((-0.5, 2.0, -0.5), (0.5, 2.0, -0.5), (-0.5, 2.0, 0.5), (6, 6, 6)))
+    s.set_camera((0, 1.5, -3), (0, 0, 0), 50)
+    s.set_options(width=48, height=32, max_depth=3, enable_rr=0)
+    return s.build()
+
+
+def test_consistent_estimator_sees_lights_from_behind_one_sided_surfaces():
+    """The reference offsets shadow-ray origins along the unflipped normal
+    (pathtrace.glsl:130): seen from behind, a one-sided floor shadows its own
+    light samples.  Estimator 1 offsets to the incident side, so flipping the
+    floor's winding leaves its image unchanged; estimator 0 keeps the
+    reference behaviour.  Both match the oracle path by path."""
+    print("[guide] one-sided surfaces", flush=True)
+    means = {}
+    for est in (0, 1):
+        for flipped in (False, True):
+            sc = _floor_scene(flipped)
+            ctx = lfh.Context.from_scene(sc, estimator=est)
+            cam = sc.camera()
+            ctx.update(cam, reset=True)
+            ctx.render_passes(256)
+            means[est, flipped] = ctx.output_hdr().mean()
+            orc = native.OracleScene(sc.arrays())
+            opts = sc.options()
+            opts.estimator = est
+            o = native.options_from(opts, cam, lights=True, envmap=False)
+            rng = np.random.default_rng(7)
+            n = 20000
+            rays = np.zeros((n, 6), np.float32)
+            rays[:, 0:3] = cam.position
+            dd = rng.normal(size=(n, 3)) * 0.2 + np.asarray(cam.forward)
+            rays[:, 3:6] = dd / np.linalg.norm(dd, axis=1, keepdims=True)
+            seeds = rng.integers(0, 2**32, (n, 4), dtype=np.uint64).astype(np.uint32)
+            g = ctx.path_trace_rays(rays, seeds)
+            c = orc.path_trace_rays(o, rays, seeds, 32)
+            assert np.isclose(g, c, rtol=1e-4, atol=1e-5).all(1).mean() > 0.99, (est, flipped)
+    assert abs(means[1, True] - means[1, False]) < 0.03 * means[1, False], means
+    assert means[0, True] < 0.5 * means[0, False], means
