@@ -472,6 +472,8 @@ def guiding_leg(args):
     plain_ctx.set_stream(stream.cuda_stream)
     plain = _converge(plain_ctx, cam, ref, ref_var, stream, args.guide_max_spp, target)
     log(f"plain: {plain}")
+    npix = sc.options().width * sc.options().height
+    plain_rays = [round(r / (npix * plain[-1][0]), 3) for r in plain_ctx.ray_counts()]
     plain_ctx.close()
     g_ctx = lfh.Context.from_scene(sc, estimator=1, guiding=1)
     g_ctx.set_stream(stream.cuda_stream)
@@ -479,6 +481,7 @@ def guiding_leg(args):
     g_ctx.set_guide(guide, train_every=1)
     guided = _converge(g_ctx, cam, ref, ref_var, stream, args.guide_max_spp, target)
     log(f"guided: {guided} (Adam steps {guide.steps()}, samples {g_ctx.guide_samples_seen()})")
+    guided_rays = [round(r / (npix * guided[-1][0]), 3) for r in g_ctx.ray_counts()]
     tp, xp = _time_to(plain, target)
     tg, xg = _time_to(guided, target)
     return {"metric": "wall time to relative RMSE 1e-2 (16x16 block means), guided / plain (consistent estimator)",
@@ -487,7 +490,12 @@ def guiding_leg(args):
             "plain_curve": [[s, round(t, 4), round(e, 6)] for s, t, e in plain],
             "guided_curve": [[s, round(t, 4), round(e, 6)] for s, t, e in guided],
             "reference": {"spp": 2 * half, "seconds": round(t_ref, 1), "mean_radiance": round(float(ref.mean()), 5),
-                          "noise_rms": round(math.sqrt(ref_var), 6)},
+                          "block_noise_rms": round(math.sqrt(ref_var), 6),
+                          "block_noise_rms_rel": round(math.sqrt(ref_var) / float(ref.mean()), 6)},
+            "rays_per_sample": {"plain": {"extended": plain_rays[0], "shadow": plain_rays[1]},
+                                "guided": {"extended": guided_rays[0], "shadow": guided_rays[1]}},
+            "ms_per_pass": {"plain": round(1e3 * plain[-1][1] / plain[-1][0], 3),
+                            "guided": round(1e3 * guided[-1][1] / guided[-1][0], 3)},
             "adam_steps": guide.steps(), "training_samples": g_ctx.guide_samples_seen(),
             "dtype": "f32 (MLP bf16/f32-acc)", "data": "synthetic",
             "config": {"workload": "guiding room ~200k triangles, 1280x720, depth 10, door 5% of wall",

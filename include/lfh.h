@@ -130,6 +130,18 @@ typedef struct lfh_options {
     int32_t reserved[5];
 } lfh_options;
 
+/* Post-process options: the RenderOptions fields behind postprocess.glsl's
+ * uniforms (Renderer.h:18-67, set at TiledRenderer.cpp:541-552). */
+typedef struct lfh_postprocess {
+    int32_t tonemap_index;           /* 0 none, 1 linear curve, 2 ACES, 3 Reinhard, 4 Kanjero,
+                                      * 5 Hejl-Dawson, 6 Uncharted 2 (postprocess.glsl:130-162) */
+    int32_t use_ca;                  /* chromatic aberration (postprocess.glsl:96-116)      */
+    int32_t use_ca_distortion;
+    float   ca_distance, ca_p1, ca_p2, ca_p3;
+    int32_t use_vignette;            /* postprocess.glsl:119-122, 164-167                    */
+    float   vignette_intensity, vignette_power;
+} lfh_postprocess;
+
 /* Camera uniforms (TiledRenderer.cpp:507-513); fov in radians. */
 typedef struct lfh_camera {
     float position[3];
@@ -174,6 +186,9 @@ int  lfh_scene_set_hdr(lfh_scene* s, const float* rgb, int32_t width, int32_t he
 int  lfh_scene_set_options(lfh_scene* s, const lfh_options* opt);
 int  lfh_scene_get_options(const lfh_scene* s, lfh_options* opt);
 int  lfh_scene_get_camera(const lfh_scene* s, lfh_camera* cam);
+/* Post-process options of a loaded scene (Renderer block of the .lfs,
+ * Loader.cpp:262-282; defaults Renderer.h:20-44). */
+int  lfh_scene_get_postprocess(const lfh_scene* s, lfh_postprocess* pp);
 /* CreateAccelerationStructures (Scene.cpp:287-338). */
 int  lfh_scene_build(lfh_scene* s);
 /* RebuildInstances (Scene.cpp:272-285) after lfh_scene_set_instance_transform. */
@@ -222,6 +237,15 @@ int  lfh_get_progress(const lfh_ctx* ctx, float* out);
 /* Linear HDR accum / samples, RGB32F, bottom-up rows like GL; dst is caller-
  * allocated w*h*3 floats and w, h must equal the resolution. */
 int  lfh_get_output_hdr(lfh_ctx* ctx, float* dst, int32_t w, int32_t h);
+/* The post-processed image (postprocess.glsl over the accumulation scaled by
+ * inv_samples; inv_samples <= 0: 1 / sample counter), RGB float rows bottom-up:
+ * what GetOutputBufferHDR reads (TiledRenderer.cpp:399-414); and its RGB8
+ * read-back, round(255 clamp(c)), what GetOutputBuffer reads
+ * (TiledRenderer.cpp:382-397).  Caller-allocated w*h*3 buffers. */
+int  lfh_get_output_post(lfh_ctx* ctx, const lfh_postprocess* pp, float inv_samples, float* dst,
+                         int32_t w, int32_t h);
+int  lfh_get_output_ldr(lfh_ctx* ctx, const lfh_postprocess* pp, float inv_samples, uint8_t* dst,
+                        int32_t w, int32_t h);
 /* Raw accumulation sum (RGB, w*h*3) and its Kahan compensation (may be NULL). */
 int  lfh_get_accum(lfh_ctx* ctx, float* sum, float* comp, int32_t w, int32_t h);
 /* Render `passes` full passes back to back (each = Update(no reset) + Render

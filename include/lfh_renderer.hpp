@@ -14,11 +14,11 @@
 //                          else advances tile / sample / frame counters
 //   Render()               TiledRenderer::Render: one tile, one sample/pixel
 //   GetSampleCount(), GetProgress()
-//   GetOutputBufferHDR()   linear accum / samples, RGB float, bottom-up rows;
-//                          allocates with new[] like the reference (the caller
-//                          frees with delete[])
-//   GetOutputBuffer()      RGB8 of the same linear image, clamped (no tonemap;
-//                          tonemapping is display-only and not part of the core)
+//   GetOutputBufferHDR()   the post-processed image (postprocess.glsl: accum /
+//                          samples, tonemap, chromatic aberration, vignette),
+//                          RGB float, bottom-up rows; allocates with new[] like
+//                          the reference (the caller frees with delete[])
+//   GetOutputBuffer()      its RGB8 read-back, round(255 clamp(c))
 //   Present(), SetViewport(), Denoise()  headless no-ops
 #pragma once
 
@@ -93,22 +93,39 @@ public:
         return n;
     }
 
+    // The post-processed image (postprocess.glsl with the scene's tonemap,
+    // chromatic aberration and vignette options) as RGB float.
     virtual void GetOutputBufferHDR(float** data, int& w, int& h) {
         w = width_;
         h = height_;
         *data = new float[(size_t)w * h * 3];
-        check(lfh_get_output_hdr(ctx_, *data, w, h), "GetOutputBufferHDR");
+        lfh_postprocess pp = postprocess();
+        check(lfh_get_output_post(ctx_, &pp, 0.0f, *data, w, h), "GetOutputBufferHDR");
     }
 
+    // Its RGB8 read-back.
     virtual void GetOutputBuffer(unsigned char** data, int& w, int& h) {
-        float* hdr = nullptr;
-        GetOutputBufferHDR(&hdr, w, h);
+        w = width_;
+        h = height_;
         *data = new unsigned char[(size_t)w * h * 3];
-        for (size_t i = 0; i < (size_t)w * h * 3; ++i) {
-            float v = hdr[i] < 0.f ? 0.f : (hdr[i] > 1.f ? 1.f : hdr[i]);
-            (*data)[i] = (unsigned char)(v * 255.f + 0.5f);
-        }
-        delete[] hdr;
+        lfh_postprocess pp = postprocess();
+        check(lfh_get_output_ldr(ctx_, &pp, 0.0f, *data, w, h), "GetOutputBuffer");
+    }
+
+    lfh_postprocess postprocess() const {
+        const auto& r = scene_->renderOptions;
+        lfh_postprocess p;
+        p.tonemap_index = r.tonemapIndex;
+        p.use_ca = r.useCA ? 1 : 0;
+        p.use_ca_distortion = r.useCADistortion ? 1 : 0;
+        p.ca_distance = r.caDistance;
+        p.ca_p1 = r.caP1;
+        p.ca_p2 = r.caP2;
+        p.ca_p3 = r.caP3;
+        p.use_vignette = r.useVignette ? 1 : 0;
+        p.vignette_intensity = r.vignetteIntensity;
+        p.vignette_power = r.vignettePower;
+        return p;
     }
 
     virtual uint32_t SetViewport(int, int) { return 0; }

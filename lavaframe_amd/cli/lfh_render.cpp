@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "../csrc/host_scene.h"
 #include "lfh.h"
@@ -71,11 +72,11 @@ int main(int argc, char** argv) {
         lfh_synchronize(renderer.context());
         double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         if (debug) std::printf("%d samples in %.3f s\n", max_samples, s);
-        float* hdr = nullptr;
-        int w = 0, h = 0;
-        // export after maxSamples complete passes: accum / maxSamples
-        renderer.GetOutputBufferHDR(&hdr, w, h);
-        float scale = (float)(max_samples + 1) / (float)max_samples;
+        // export after maxSamples complete passes: the post-processed image of
+        // accum / maxSamples (the sample counter already reads maxSamples + 1)
+        const int w = scene.renderOptions.resolution.x, h = scene.renderOptions.resolution.y;
+        lfh_postprocess pp = renderer.postprocess();
+        const float inv = 1.0f / (float)max_samples;
         if (name.empty()) {
             size_t p = scene_file.find_last_of("/\\");
             name = scene_file.substr(p == std::string::npos ? 0 : p + 1);
@@ -83,29 +84,35 @@ int main(int argc, char** argv) {
             if (d != std::string::npos) name = name.substr(0, d);
         }
         std::string path = out_path + (out_path.empty() || out_path.back() == '/' ? "" : "/") + name + "." + type;
+        std::vector<float> hdr;
+        std::vector<unsigned char> ldr;
+        int rc;
+        if (type == "ppm") {
+            ldr.resize((size_t)w * h * 3);
+            rc = lfh_get_output_ldr(renderer.context(), &pp, inv, ldr.data(), w, h);
+        } else {
+            hdr.resize((size_t)w * h * 3);
+            rc = lfh_get_output_post(renderer.context(), &pp, inv, hdr.data(), w, h);
+        }
+        if (rc != LFH_OK) {
+            std::fprintf(stderr, "export: %s\n", lfh_last_error(renderer.context()));
+            return 1;
+        }
         FILE* f = std::fopen(path.c_str(), "wb");
         if (!f) {
             std::fprintf(stderr, "cannot write %s\n", path.c_str());
-            delete[] hdr;
             return 1;
         }
         if (type == "ppm") {
+            // PPM rows are top-to-bottom
             std::fprintf(f, "P6\n%d %d\n255\n", w, h);
-            for (int y = h - 1; y >= 0; --y)
-                for (int x = 0; x < w; ++x)
-                    for (int c = 0; c < 3; ++c) {
-                        float v = hdr[((size_t)y * w + x) * 3 + c] * scale;
-                        v = v < 0.f ? 0.f : (v > 1.f ? 1.f : v);
-                        std::fputc((int)(v * 255.f + 0.5f), f);
-                    }
+            for (int y = h - 1; y >= 0; --y) std::fwrite(&ldr[(size_t)y * w * 3], 1, (size_t)w * 3, f);
         } else {
             // PFM rows are bottom-to-top, like the GL readback
             std::fprintf(f, "PF\n%d %d\n-1.0\n", w, h);
-            for (size_t i = 0; i < (size_t)w * h * 3; ++i) hdr[i] *= scale;
-            std::fwrite(hdr, sizeof(float), (size_t)w * h * 3, f);
+            std::fwrite(hdr.data(), sizeof(float), hdr.size(), f);
         }
         std::fclose(f);
-        delete[] hdr;
         std::printf("%s\n", path.c_str());
     } catch (const std::exception& e) {
         std::fprintf(stderr, "%s\n", e.what());

@@ -11,6 +11,7 @@
 #include "capi_common.h"
 #include "ctx.h"
 #include "kernels.h"
+#include "post_kernels.h"
 #include "pt_kernels.h"
 #include "render.h"
 
@@ -80,8 +81,11 @@ int render_init(lfh_ctx* c) {
 
 void render_release(lfh_ctx*) {}
 
-// Paths per wavefront targeted by multi-sample passes over small frames.
-constexpr int64_t kBatchPaths = 1 << 21;
+// Paths per wavefront targeted by multi-sample passes over one-tile frames
+// (292 B of path state each, 2.4 GB at this size).  The last bounces of a
+// wavefront hold few paths and cost a traversal's memory latency per launch,
+// so several passes share them.
+constexpr int64_t kBatchPaths = 1 << 23;
 constexpr int kMaxBatch = 64;
 
 // One path-traced pass over the current tile (TiledRenderer::Render's
@@ -122,7 +126,7 @@ int render_tile(lfh_ctx* c, int batch) {
     RCHK(c, pt_generate(cp, lfh::view(pb), pb.rq0.as<float4>(), cnt + 0, c->stream), "render: generate");
     GuideHook gh{};
     lfhd::GuideDev gd{};
-    if (guide_pass_begin(c, &gh, &gd) < 0) return rfail(c, LFH_E_HIP, "render: guide records");
+    if (guide_pass_begin(c, batch, &gh, &gd) < 0) return rfail(c, LFH_E_HIP, "render: guide buffers");
     for (int depth = 0; depth < c->opt.max_depth; ++depth) {
         const int cur = depth & 1;
         float4* q[2] = {pb.rq0.as<float4>(), pb.rq1.as<float4>()};
@@ -134,7 +138,7 @@ int render_tile(lfh_ctx* c, int batch) {
     }
     RCHK(c, pt_accumulate(lfh::view(pb), cp, c->rs.accum.as<float4>(), c->rs.comp.as<float4>(), c->opt.kahan, c->stream),
          "render: accumulate");
-    int gr = guide_pass_end(c, gh);
+    int gr = guide_pass_end(c, gh, batch);
     if (gr != LFH_OK) return rfail(c, gr, std::string("render: ") + lfh_internal::thread_error());
     return LFH_OK;
 }
@@ -188,12 +192,13 @@ int lfh_render_passes(lfh_ctx* c, int32_t passes) {
     if (!c || passes < 0) return rfail(c, LFH_E_INVALID, "lfh_render_passes: bad argument");
     if (!c->has_cam) return rfail(c, LFH_E_STATE, "lfh_render_passes: lfh_update has not been called");
     const int tiles = c->numTilesX * c->numTilesY;
-    if (tiles == 1 && !c->guide) {
-        // one-tile frames: consecutive passes share one wavefront
-        int kmax = (int)std::min<int64_t>(lfh::kMaxBatch, std::max<int64_t>(1, lfh::kBatchPaths / c->rs.npaths));
+    if (tiles == 1) {
+        // one-tile frames: consecutive passes share one wavefront (with a guide,
+        // once it has finished warming up)
+        const int kmax = (int)std::min<int64_t>(lfh::kMaxBatch, std::max<int64_t>(1, lfh::kBatchPaths / c->rs.npaths));
         int done = 0;
         while (done < passes) {
-            const int k = std::min(kmax, passes - done);
+            const int k = std::min({kmax, passes - done, lfh::guide_batch_limit(c)});
             lfh::advance_counters(c, false);
             const int frame0 = c->frameCounter;
             for (int i = 1; i < k; ++i) lfh::advance_counters(c, false);  // frames frame0 .. frame0 + k - 1
@@ -263,6 +268,43 @@ int lfh_get_output_hdr(lfh_ctx* c, float* dst, int32_t w, int32_t h) {
     RCHK(c, hipMemcpyAsync(dst, c->out.p, bytes, hipMemcpyDeviceToHost, c->stream), "lfh_get_output_hdr: copy");
     RCHK(c, hipStreamSynchronize(c->stream), "lfh_get_output_hdr: sync");
     return LFH_OK;
+}
+
+static int get_post(lfh_ctx* c, const lfh_postprocess* pp, float inv, float* rgb, uint8_t* ldr, int32_t w,
+                    int32_t h, const char* fn) {
+    if (!c || !pp || (!rgb && !ldr)) return rfail(c, LFH_E_INVALID, std::string(fn) + ": NULL argument");
+    if (w != c->opt.width || h != c->opt.height) return rfail(c, LFH_E_INVALID, std::string(fn) + ": size mismatch");
+    if (pp->tonemap_index < 0 || pp->tonemap_index > 6)
+        return rfail(c, LFH_E_INVALID, std::string(fn) + ": tonemap_index must be 0..6");
+    lfhd::PostParams p;
+    p.tonemap_index = pp->tonemap_index;
+    p.use_ca = pp->use_ca != 0;
+    p.use_ca_distortion = pp->use_ca_distortion != 0;
+    p.ca_distance = pp->ca_distance;
+    p.ca_p1 = pp->ca_p1;
+    p.ca_p2 = pp->ca_p2;
+    p.ca_p3 = pp->ca_p3;
+    p.use_vignette = pp->use_vignette != 0;
+    p.vignette_intensity = pp->vignette_intensity;
+    p.vignette_power = pp->vignette_power;
+    if (!(inv > 0.0f)) inv = 1.0f / (float)c->sampleCounter;
+    const size_t n = (size_t)c->rs.npix * 3;
+    const size_t bytes = rgb ? n * sizeof(float) : n;
+    if (c->out.bytes < bytes) RCHK(c, c->out.alloc(bytes), fn);
+    RCHK(c, lfhd::post_process(c->rs.accum.as<float4>(), c->rs.comp.as<float4>(), w, h, inv, p,
+                               rgb ? c->out.as<float>() : nullptr, rgb ? nullptr : c->out.as<uint8_t>(), c->stream),
+         fn);
+    RCHK(c, hipMemcpyAsync(rgb ? (void*)rgb : (void*)ldr, c->out.p, bytes, hipMemcpyDeviceToHost, c->stream), fn);
+    RCHK(c, hipStreamSynchronize(c->stream), fn);
+    return LFH_OK;
+}
+
+int lfh_get_output_post(lfh_ctx* c, const lfh_postprocess* pp, float inv, float* dst, int32_t w, int32_t h) {
+    return get_post(c, pp, inv, dst, nullptr, w, h, "lfh_get_output_post");
+}
+
+int lfh_get_output_ldr(lfh_ctx* c, const lfh_postprocess* pp, float inv, uint8_t* dst, int32_t w, int32_t h) {
+    return get_post(c, pp, inv, nullptr, dst, w, h, "lfh_get_output_ldr");
 }
 
 int lfh_get_accum(lfh_ctx* c, float* sum, float* comp, int32_t w, int32_t h) {

@@ -144,6 +144,22 @@ int lfh_scene_get_options(const lfh_scene* s, lfh_options* o) {
     return LFH_OK;
 }
 
+int lfh_scene_get_postprocess(const lfh_scene* s, lfh_postprocess* pp) {
+    if (!s || !pp) return fail_scene(nullptr, LFH_E_INVALID, "lfh_scene_get_postprocess: NULL argument");
+    const auto& r = s->scene.renderOptions;
+    pp->tonemap_index = r.tonemapIndex;
+    pp->use_ca = r.useCA;
+    pp->use_ca_distortion = r.useCADistortion;
+    pp->ca_distance = r.caDistance;
+    pp->ca_p1 = r.caP1;
+    pp->ca_p2 = r.caP2;
+    pp->ca_p3 = r.caP3;
+    pp->use_vignette = r.useVignette;
+    pp->vignette_intensity = r.vignetteIntensity;
+    pp->vignette_power = r.vignettePower;
+    return LFH_OK;
+}
+
 int lfh_scene_get_camera(const lfh_scene* s, lfh_camera* c) {
     if (!s || !c) return fail_scene(nullptr, LFH_E_INVALID, "lfh_scene_get_camera: NULL argument");
     if (!s->scene.camera) return fail_scene(nullptr, LFH_E_STATE, "lfh_scene_get_camera: scene has no camera");

@@ -132,8 +132,7 @@ struct GuideBufs {
     DevBuf samples;                // compacted training samples, 4 float4 each
     DevBuf block_counts, block_offsets, block_wsum, partial;
     DevBuf total;                  // int sample count, then float weight sum
-    int stride = 1;                // record paths p with p % stride == phase
-    int nslots = 0;                // recorded paths x max_depth
+    int nslots = 0;                // record slots: recorded paths x max_depth, largest wavefront
 };
 
 hipError_t upload_scene(const PackedScene& ps, DeviceScene* ds, hipStream_t s);
@@ -171,6 +170,5 @@ struct lfh_ctx {
     struct lfh_guide* guide = nullptr;
     int guide_train_every = 0;
     int guide_pass = 0;            // passes rendered with the guide attached
-    int64_t guide_samples = 0;     // training samples collected so far
     lfh::GuideBufs gbuf;
 };

@@ -7,6 +7,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <limits>
 #include <vector>
 
 #include "../../include/lfh.h"
@@ -23,7 +24,7 @@ using lfhd::kGuideParams;
 namespace {
 constexpr int kBatch = 65536;             // samples per optimisation step
 constexpr int kMaxSamples = 4 * kBatch;   // samples kept from one training pass
-constexpr int kRecordedPaths = 1 << 20;   // target number of recorded paths per pass
+constexpr int kRecordedPaths = 1 << 18;   // target number of recorded paths per pass
 constexpr int kWarmSteps = 256;           // Adam steps taken at the warm-up rate
 constexpr int kWarmRepeats = 4;           // optimisation passes over a pass's samples while warming up
 constexpr float kWeightCap = 32.0f;       // online training: weights / their mean, capped
@@ -187,49 +188,87 @@ int run_batch(lfh_guide* g, int64_t n, bool step, float* loss, float* grads) {
 namespace lfh {
 
 // Per-pass hooks of the renderer ------------------------------------------------------
-int guide_pass_begin(lfh_ctx* c, GuideHook* gh, lfhd::GuideDev* gd) {
+// Passes that may share one wavefront: one at a time while a training guide is
+// young (each pass then samples the guide the previous pass trained), any
+// number afterwards.
+int guide_batch_limit(const lfh_ctx* c) {
+    const lfh_guide* g = c->guide;
+    if (g && c->guide_train_every > 0 && g->steps < kWarmSteps) return 1;
+    return std::numeric_limits<int>::max();
+}
+
+namespace {
+// Training passes among the wavefront's passes guide_pass .. guide_pass + batch - 1.
+int training_passes(const lfh_ctx* c, int batch) {
+    if (c->guide_train_every <= 0 || c->opt.max_depth <= 0) return 0;
+    int n = 0;
+    for (int i = 0; i < batch; ++i) n += ((c->guide_pass + i) % c->guide_train_every) == 0;
+    return n;
+}
+
+// Recorded paths: every stride-th of the wavefront's paths, at most kRecordedPaths.
+struct RecordPlan {
+    int64_t paths;
+    int stride, recorded, nslots;
+};
+RecordPlan record_plan(const lfh_ctx* c, int batch) {
+    RecordPlan r;
+    r.paths = (int64_t)c->rs.npaths * batch;
+    r.stride = (int)std::max<int64_t>(1, (r.paths + kRecordedPaths - 1) / kRecordedPaths);
+    r.recorded = (int)((r.paths + r.stride - 1) / r.stride);
+    r.nslots = r.recorded * std::max(1, c->opt.max_depth);
+    return r;
+}
+}  // namespace
+
+int guide_pass_begin(lfh_ctx* c, int batch, GuideHook* gh, lfhd::GuideDev* gd) {
     *gh = GuideHook{};
     lfh_guide* g = c->guide;
     if (!g) return 0;
     GuideBufs& b = c->gbuf;
+    // lobes per hit-queue entry: the queues grow with multi-pass wavefronts
+    const size_t lobe_bytes = (size_t)c->pass.cap * kGuideLobeF4 * sizeof(float4);
+    if (b.lobes.bytes < lobe_bytes) {
+        if (hipStreamSynchronize(c->stream) != hipSuccess || b.lobes.alloc(lobe_bytes) != hipSuccess) return -1;
+    }
     gh->enabled = 1;
     gh->lobes = b.lobes.as<const float4>();
     gh->max_depth = c->opt.max_depth;
     *gd = g->dev();
-    const bool train = c->guide_train_every > 0 && (c->guide_pass % c->guide_train_every) == 0;
-    if (train && c->opt.max_depth > 0) {
+    if (training_passes(c, batch) > 0) {
+        const RecordPlan rp = record_plan(c, batch);
         gh->train = 1;
-        gh->stride = b.stride;
-        gh->phase = (c->guide_pass / c->guide_train_every) % b.stride;
+        gh->stride = rp.stride;
+        gh->phase = (c->guide_pass / c->guide_train_every) % rp.stride;
         gh->rec = b.rec.as<float4>();
-        if (pt_clear4(b.rec.as<float4>(), (int64_t)b.nslots * 4, c->stream) != hipSuccess) return -1;
+        if (pt_clear4(b.rec.as<float4>(), (int64_t)rp.nslots * 4, c->stream) != hipSuccess) return -1;
     }
     return 1;
 }
 
-int guide_pass_end(lfh_ctx* c, const GuideHook& gh) {
+int guide_pass_end(lfh_ctx* c, const GuideHook& gh, int batch) {
     lfh_guide* g = c->guide;
-    c->guide_pass++;
+    const int ntrain = training_passes(c, batch);
+    c->guide_pass += batch;
     if (!g || !gh.train) return LFH_OK;
     GuideBufs& b = c->gbuf;
-    hipError_t e = lfhd::guide_collect(b.rec.as<float4>(), b.nslots, c->opt.max_depth, gh.stride, gh.phase,
-                                       c->pass.rad.as<float4>(), c->rs.npaths, b.block_counts.as<int>(),
+    const RecordPlan rp = record_plan(c, batch);
+    // collect, then train on the device-side sample count: no host round trip
+    hipError_t e = lfhd::guide_collect(b.rec.as<float4>(), rp.nslots, c->opt.max_depth, gh.stride, gh.phase,
+                                       c->pass.rad.as<float4>(), (int)rp.paths, b.block_counts.as<int>(),
                                        b.block_offsets.as<int>(), b.total.as<int>(), b.samples.as<float4>(),
-                                       kMaxSamples, b.block_wsum.as<float>(), b.total.as<float>() + 1, c->stream);
-    int total = 0;
-    if (e == hipSuccess) e = hipMemcpyAsync(&total, b.total.p, sizeof(int), hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+                                       kMaxSamples, b.block_wsum.as<float>(), b.total.as<float>() + 1,
+                                       reinterpret_cast<unsigned long long*>(b.total.as<char>() + 8), c->stream);
     if (e != hipSuccess) return ghip(g, e, "guide: collect");
-    c->guide_samples += total;
-    // one Adam step per batch of the pass's samples; while the guide is young,
-    // several passes over them
-    const int reps = g->steps < kWarmSteps ? kWarmRepeats : 1;
-    for (int r = 0; r < reps && total > 0; ++r) {
-        for (int start = 0; start < total; start += kBatch) {
+    // one Adam step per training pass on a strided batch of the wavefront's
+    // samples; several while the guide is young
+    for (int t = 0; t < ntrain; ++t) {
+        const int reps = g->steps < kWarmSteps ? kWarmRepeats : 1;
+        for (int r = 0; r < reps; ++r) {
             lfhd::GuideAdam ad = g->adam(true);
-            e = lfhd::guide_train(g->dev(), b.samples.as<float4>(), b.total.as<int>(), start, kBatch,
-                                  b.partial.as<float>(), ad, nullptr, nullptr, c->stream, b.total.as<float>() + 1,
-                                  kWeightCap);
+            e = lfhd::guide_train(g->dev(), b.samples.as<float4>(), b.total.as<int>(), (int)(g->steps & 0x7fffffff),
+                                  kBatch, b.partial.as<float>(), ad, nullptr, nullptr, c->stream,
+                                  b.total.as<float>() + 1, kWeightCap);
             if (e != hipSuccess) return ghip(g, e, "guide: train");
             g->steps++;
         }
@@ -394,11 +433,8 @@ int lfh_set_guide(lfh_ctx* c, lfh_guide* g, int32_t train_every) {
     c->guide_train_every = train_every;
     if (!g) return LFH_OK;
     lfh::GuideBufs& b = c->gbuf;
-    const int npaths = c->rs.npaths;
-    b.stride = (npaths + kRecordedPaths - 1) / kRecordedPaths;
-    if (b.stride < 1) b.stride = 1;
-    const int recorded = (npaths + b.stride - 1) / b.stride;
-    b.nslots = recorded * (c->opt.max_depth > 0 ? c->opt.max_depth : 1);
+    // records for the largest wavefront: kRecordedPaths paths x max_depth
+    b.nslots = kRecordedPaths * (c->opt.max_depth > 0 ? c->opt.max_depth : 1);
     const int nblocks = lfhd::guide_collect_blocks(b.nslots);
     hipError_t e = ensure(b.lobes, (size_t)c->pass.cap * kGuideLobeF4 * sizeof(float4));
     if (e == hipSuccess) e = ensure(b.rec, (size_t)b.nslots * 4 * sizeof(float4));
@@ -406,7 +442,10 @@ int lfh_set_guide(lfh_ctx* c, lfh_guide* g, int32_t train_every) {
     if (e == hipSuccess) e = ensure(b.block_counts, (size_t)(nblocks + 1) * sizeof(int));
     if (e == hipSuccess) e = ensure(b.block_offsets, (size_t)(nblocks + 1) * sizeof(int));
     if (e == hipSuccess) e = ensure(b.block_wsum, (size_t)(nblocks + 1) * sizeof(float));
-    if (e == hipSuccess) e = ensure(b.total, 64);
+    if (e == hipSuccess && !b.total.p) {
+        e = b.total.alloc(64);   // int count, float weight sum, u64 running count at byte 8
+        if (e == hipSuccess) e = hipMemsetAsync(b.total.p, 0, 64, c->stream);
+    }
     if (e == hipSuccess) e = ensure(b.partial, lfhd::guide_partial_floats(kBatch) * sizeof(float));
     if (e != hipSuccess) {
         c->guide = nullptr;
@@ -417,7 +456,14 @@ int lfh_set_guide(lfh_ctx* c, lfh_guide* g, int32_t train_every) {
 
 int lfh_guide_samples_seen(const lfh_ctx* c, int64_t* n) {
     if (!c || !n) return gfail(nullptr, LFH_E_INVALID, "lfh_guide_samples_seen: NULL argument");
-    *n = c->guide_samples;
+    unsigned long long v = 0;
+    if (c->gbuf.total.p) {
+        if (hipMemcpyAsync(&v, c->gbuf.total.as<char>() + 8, sizeof v, hipMemcpyDeviceToHost, c->stream) !=
+                hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess)
+            return gfail(nullptr, LFH_E_HIP, "lfh_guide_samples_seen: copy");
+    }
+    *n = (int64_t)v;
     return LFH_OK;
 }
 

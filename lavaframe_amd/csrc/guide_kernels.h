@@ -33,8 +33,9 @@ hipError_t guide_sample(const float4* lobes, const uint4* seeds, int n, float* d
 hipError_t guide_pdf(const float4* lobes, const float* dirs, int n, float* pdf, hipStream_t s);
 int guide_train_blocks(int batch);
 size_t guide_partial_floats(int batch);
-// One batch: samples [start, start + batch) of the device sample list (count
-// valid).  grad_out != NULL: store the gradient only; else one Adam step.
+// One batch of at most `batch` samples of the device sample list (count
+// valid): every stride-th sample, stride = ceil(count / batch), from
+// start % stride.  grad_out != NULL: store the gradient only; else one Adam step.
 // wsum != NULL: sample weights are normalised by their mean (*wsum / *count)
 // and capped at wcap.
 hipError_t guide_train(const GuideDev& g, const float4* samples, const int* count, int start, int batch,
@@ -43,7 +44,8 @@ hipError_t guide_train(const GuideDev& g, const float4* samples, const int* coun
 hipError_t guide_refresh(const float* master, uint16_t* wf, uint16_t* wb, float* bias, hipStream_t s);
 hipError_t guide_collect(const float4* rec, int nslots, int max_depth, int stride, int phase, const float4* rad,
                          int npaths, int* block_counts, int* block_offsets, int* total, float4* samples,
-                         int max_samples, float* block_wsum, float* wsum, hipStream_t s);
+                         int max_samples, float* block_wsum, float* wsum, unsigned long long* seen,
+                         hipStream_t s);
 int guide_collect_blocks(int nslots);
 
 }  // namespace lfhd

@@ -922,7 +922,7 @@ bool LoadSceneFromFile(const std::string& filename, Scene* scene, RenderOptions&
         if (std::strstr(line, "Renderer")) {
             char envMap[200] = "None";
             char enableRR[10] = "None";
-            int useCA = 0;
+            int useCA = ro.useCA ? 1 : 0, useCAD = ro.useCADistortion ? 1 : 0, useVig = ro.useVignette ? 1 : 0;
             while (std::fgets(line, sizeof line, file)) {
                 if (std::strchr(line, '}')) break;
                 std::sscanf(line, " envMap %199s", envMap);
@@ -939,8 +939,21 @@ bool LoadSceneFromFile(const std::string& filename, Scene* scene, RenderOptions&
                 std::sscanf(line, " useAces %i", &legacyAces);
                 if (legacyAces == 1) ro.tonemapIndex = 1;
                 if (legacyAces == 0) ro.tonemapIndex = 0;
+                // Loader.cpp:272-282 reads these flags with %i into bool fields;
+                // here each is read as an int and kept as (value != 0)
                 std::sscanf(line, " useChromaticAberration %i", &useCA);
+                std::sscanf(line, " CADistance %f", &ro.caDistance);
+                std::sscanf(line, " useCAdistortion %i", &useCAD);
+                std::sscanf(line, " CAAngularity %f", &ro.caP1);
+                std::sscanf(line, " CADirectionality %f", &ro.caP2);
+                std::sscanf(line, " CACenter %f", &ro.caP3);
+                std::sscanf(line, " useVignette %i", &useVig);
+                std::sscanf(line, " vignetteIntensity %f", &ro.vignetteIntensity);
+                std::sscanf(line, " vignettePower %f", &ro.vignettePower);
             }
+            ro.useCA = useCA != 0;
+            ro.useCADistortion = useCAD != 0;
+            ro.useVignette = useVig != 0;
             if (std::strcmp(envMap, "None") != 0) {
                 // Radiance .hdr parsing is outside this build; the env map
                 // stays disabled, as when the reference fails to load it

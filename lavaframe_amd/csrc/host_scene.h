@@ -246,6 +246,18 @@ struct RenderOptions {
     float hdrMultiplier = 1.f;
     Vec3 bgColor{0.5f, 0.5f, 0.5f};
     int tonemapIndex = 2;
+    // post-process (Renderer.h:34-43); the reference leaves the three vignette
+    // fields uninitialised (its constructor declares locals of the same
+    // names), here they take the values it evidently intended
+    bool useCA = false;
+    bool useCADistortion = false;
+    bool useVignette = false;
+    float caDistance = 0.05f;
+    float caP1 = 5.0f;
+    float caP2 = -0.5f;
+    float caP3 = 0.5f;
+    float vignetteIntensity = 0.0f;
+    float vignettePower = 1.0f;
 };
 
 // Flat node of the reference layout (bvh_translator.h:48-53).

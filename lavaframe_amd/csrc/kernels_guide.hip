@@ -222,9 +222,18 @@ struct TrainArgs {
     float wcap;            // with wsum: weights are w / mean(w), capped at wcap
 };
 
+// The batch takes every stride-th sample of the list, stride = ceil(count /
+// batch), starting at start % stride: a batch spreads over the whole list (the
+// list is in path order), and successive starts visit all of it.
+LFH_DEV int batch_stride(const TrainArgs& t) {
+    const int c = *t.count;
+    return c <= t.batch ? 1 : (c + t.batch - 1) / t.batch;
+}
+LFH_DEV int batch_phase(const TrainArgs& t) { return t.start % batch_stride(t); }
 LFH_DEV int batch_size(const TrainArgs& t) {
-    int n = *t.count - t.start;
-    return n < 0 ? 0 : (n > t.batch ? t.batch : n);
+    const int c = *t.count, st = batch_stride(t), ph = batch_phase(t);
+    const int n = c > ph ? (c - ph + st - 1) / st : 0;
+    return n > t.batch ? t.batch : n;
 }
 
 __global__ __launch_bounds__(kTrainBlock) void k_guide_train(GuideDev g, TrainArgs ta) {
@@ -251,7 +260,7 @@ __global__ __launch_bounds__(kTrainBlock) void k_guide_train(GuideDev g, TrainAr
         const bool ok = base + s < N;
         uint16_t* row = sA[0] + s * kPitch;
         if (ok) {
-            const float4* sp = ta.samples + (size_t)(ta.start + base + s) * 4;
+            const float4* sp = ta.samples + ((size_t)(base + s) * batch_stride(ta) + batch_phase(ta)) * 4;
             const float4 a = sp[0], b = sp[1], c = sp[2], d = sp[3];
             sD[s][0] = d.x;
             sD[s][1] = d.y;
@@ -449,13 +458,31 @@ struct AdamArgs {
     float c1, c2;   // 1 / (1 - b1^t), 1 / (1 - b2^t)
 };
 
-__global__ void k_guide_reduce_adam(TrainArgs ta, AdamArgs ad, float* __restrict__ grad_out, float* __restrict__ loss_out) {
+// Stage 1: chunk r of the active blocks' partials, summed in block order.
+constexpr int kReduceChunks = 32;
+__global__ void k_guide_reduce1(TrainArgs ta, float* __restrict__ chunks) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j > kGuideParams) return;
+    const int N = batch_size(ta);
+    const int nb = (N + kTrainRows - 1) / kTrainRows;
+    const int per = (nb + kReduceChunks - 1) / kReduceChunks;
+    const int b0 = blockIdx.y * per, b1 = min(nb, b0 + per);
+    float sum = 0.0f;
+    for (int b = b0; b < b1; ++b) sum += ta.partial[(size_t)b * kPartStride + j];
+    chunks[(size_t)blockIdx.y * kPartStride + j] = sum;
+}
+
+// Stage 2: the chunks in order; then either store the gradient (grad_out) or
+// apply one Adam step (Kingma & Ba 2015, Algorithm 1) to the FP32 master
+// weights and refresh the BF16 copies.
+__global__ void k_guide_reduce_adam(TrainArgs ta, AdamArgs ad, const float* __restrict__ chunks,
+                                    float* __restrict__ grad_out, float* __restrict__ loss_out) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j > kGuideParams) return;
     const int N = batch_size(ta);
     const int nb = (N + kTrainRows - 1) / kTrainRows;
     float sum = 0.0f;
-    for (int b = 0; b < nb; ++b) sum += ta.partial[(size_t)b * kPartStride + j];
+    for (int r = 0; r < kReduceChunks; ++r) sum += chunks[(size_t)r * kPartStride + j];
     if (j == kGuideParams) {
         if (loss_out) *loss_out = sum;
         return;
@@ -499,6 +526,7 @@ struct CollectArgs {
     int max_samples;
     float* block_wsum; // per block: sum of the weights of its samples
     float* wsum;       // sum of the weights of the kept samples
+    unsigned long long* seen;  // running count of kept samples (may be NULL)
 };
 
 LFH_DEV bool collect_one(const CollectArgs& c, int slot, float4* out, float* wout = nullptr) {
@@ -574,7 +602,11 @@ __global__ __launch_bounds__(1024) void k_guide_scan(CollectArgs c, int nblocks)
         if (t < off) sws[t] += sws[t + off];
         __syncthreads();
     }
-    if (t == 1023) *c.total = s[1023] < c.max_samples ? s[1023] : c.max_samples;
+    if (t == 1023) {
+        const int kept = s[1023] < c.max_samples ? s[1023] : c.max_samples;
+        *c.total = kept;
+        if (c.seen) *c.seen += (unsigned long long)kept;
+    }
     if (t == 0) *c.wsum = sws[0];
 }
 
@@ -627,7 +659,8 @@ hipError_t guide_pdf(const float4* lobes, const float* dirs, int n, float* pdf, 
 }
 
 int guide_train_blocks(int batch) { return nblk(batch, kTrainRows); }
-size_t guide_partial_floats(int batch) { return (size_t)guide_train_blocks(batch) * kPartStride; }
+// per-block partials, then the stage-1 chunks
+size_t guide_partial_floats(int batch) { return (size_t)(guide_train_blocks(batch) + kReduceChunks) * kPartStride; }
 
 hipError_t guide_train(const GuideDev& g, const float4* samples, const int* count, int start, int batch,
                        float* partial, const GuideAdam& adam, float* grad_out, float* loss_out, hipStream_t s,
@@ -636,8 +669,11 @@ hipError_t guide_train(const GuideDev& g, const float4* samples, const int* coun
     hipLaunchKernelGGL(k_guide_train, dim3(guide_train_blocks(batch)), dim3(kTrainBlock), 0, s, g, ta);
     AdamArgs ad{adam.master, adam.m, adam.v, adam.wf, adam.wb, adam.bias, adam.lr, adam.b1, adam.b2, adam.eps,
                 adam.c1, adam.c2};
-    hipLaunchKernelGGL(k_guide_reduce_adam, dim3(nblk(kGuideParams + 1, 256)), dim3(256), 0, s, ta, ad, grad_out,
-                       loss_out);
+    float* chunks = partial + (size_t)guide_train_blocks(batch) * kPartStride;
+    hipLaunchKernelGGL(k_guide_reduce1, dim3(nblk(kGuideParams + 1, 256), kReduceChunks), dim3(256), 0, s, ta,
+                       chunks);
+    hipLaunchKernelGGL(k_guide_reduce_adam, dim3(nblk(kGuideParams + 1, 256)), dim3(256), 0, s, ta, ad,
+                       (const float*)chunks, grad_out, loss_out);
     return hipGetLastError();
 }
 
@@ -648,9 +684,10 @@ hipError_t guide_refresh(const float* master, uint16_t* wf, uint16_t* wb, float*
 
 hipError_t guide_collect(const float4* rec, int nslots, int max_depth, int stride, int phase, const float4* rad,
                          int npaths, int* block_counts, int* block_offsets, int* total, float4* samples,
-                         int max_samples, float* block_wsum, float* wsum, hipStream_t s) {
+                         int max_samples, float* block_wsum, float* wsum, unsigned long long* seen,
+                         hipStream_t s) {
     CollectArgs c{rec, nslots, max_depth, stride, phase, rad, npaths, block_counts, block_offsets, total, samples,
-                  max_samples, block_wsum, wsum};
+                  max_samples, block_wsum, wsum, seen};
     const int nb = nblk(nslots, 256);
     if (nb <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_guide_count, dim3(nb), dim3(256), 0, s, c);

@@ -13,6 +13,7 @@ PathBufs view(const PassBufs& b);
 // Learned-guide hooks of one pass: begin fills the hook and the weights view
 // (returns 1 when a guide is attached, 0 if not, -1 on a HIP error); end
 // collects the pass's training samples and runs the optimisation steps.
-int guide_pass_begin(lfh_ctx* c, GuideHook* gh, lfhd::GuideDev* gd);
-int guide_pass_end(lfh_ctx* c, const GuideHook& gh);
+int guide_pass_begin(lfh_ctx* c, int batch, GuideHook* gh, lfhd::GuideDev* gd);
+int guide_pass_end(lfh_ctx* c, const GuideHook& gh, int batch);
+int guide_batch_limit(const lfh_ctx* c);
 }  // namespace lfh

@@ -74,6 +74,22 @@ class Camera(C.Structure):
                 ("aperture", C.c_float)]
 
 
+class Postprocess(C.Structure):
+    _fields_ = [("tonemap_index", C.c_int32), ("use_ca", C.c_int32), ("use_ca_distortion", C.c_int32),
+                ("ca_distance", C.c_float), ("ca_p1", C.c_float), ("ca_p2", C.c_float), ("ca_p3", C.c_float),
+                ("use_vignette", C.c_int32), ("vignette_intensity", C.c_float), ("vignette_power", C.c_float)]
+
+    @classmethod
+    def make(cls, tonemap_index=0, use_ca=0, use_ca_distortion=0, ca_distance=0.05, ca_p1=5.0, ca_p2=-0.5,
+             ca_p3=0.5, use_vignette=0, vignette_intensity=0.0, vignette_power=1.0):
+        """Options with the reference defaults (Renderer.h:20-44) unless given."""
+        return cls(tonemap_index, use_ca, use_ca_distortion, ca_distance, ca_p1, ca_p2, ca_p3, use_vignette,
+                   vignette_intensity, vignette_power)
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class JobDesc(C.Structure):
     _fields_ = [("scene", C.POINTER(SceneDesc)), ("options", Options), ("camera", Camera),
                 ("target_spp", C.c_int32), ("priority", C.c_int32), ("user_tag", C.c_int64)]
@@ -103,6 +119,9 @@ _SIGS = {
     "lfh_scene_set_options": (C.c_int, [_P, C.POINTER(Options)]),
     "lfh_scene_get_options": (C.c_int, [_P, C.POINTER(Options)]),
     "lfh_scene_get_camera": (C.c_int, [_P, C.POINTER(Camera)]),
+    "lfh_scene_get_postprocess": (C.c_int, [_P, C.POINTER(Postprocess)]),
+    "lfh_get_output_post": (C.c_int, [_P, C.POINTER(Postprocess), C.c_float, f32p, C.c_int32, C.c_int32]),
+    "lfh_get_output_ldr": (C.c_int, [_P, C.POINTER(Postprocess), C.c_float, u8p, C.c_int32, C.c_int32]),
     "lfh_scene_build": (C.c_int, [_P]),
     "lfh_scene_set_instance_transform": (C.c_int, [_P, C.c_int32, f32p]),
     "lfh_scene_rebuild_instances": (C.c_int, [_P]),
@@ -350,6 +369,11 @@ class Scene:
         self._chk(self.lib.lfh_scene_get_camera(self.h, C.byref(c)))
         return c
 
+    def postprocess(self) -> Postprocess:
+        p = Postprocess()
+        self._chk(self.lib.lfh_scene_get_postprocess(self.h, C.byref(p)))
+        return p
+
     def build(self):
         self._chk(self.lib.lfh_scene_build(self.h))
         return self
@@ -506,6 +530,21 @@ class Context:
         w, h = self.options.width, self.options.height
         out = np.zeros((h, w, 3), np.float32)
         self._chk(self.lib.lfh_get_output_hdr(self.h, fptr(out), w, h))
+        return out
+
+    def output_post(self, pp: Postprocess, inv_samples: float = 0.0) -> np.ndarray:
+        """Post-processed RGB float image (bottom-up rows)."""
+        w, h = self.options.width, self.options.height
+        out = np.zeros((h, w, 3), np.float32)
+        self._chk(self.lib.lfh_get_output_post(self.h, C.byref(pp), float(inv_samples), fptr(out), w, h))
+        return out
+
+    def output_ldr(self, pp: Postprocess, inv_samples: float = 0.0) -> np.ndarray:
+        """RGB8 read-back of the post-processed image (bottom-up rows)."""
+        w, h = self.options.width, self.options.height
+        out = np.zeros((h, w, 3), np.uint8)
+        self._chk(self.lib.lfh_get_output_ldr(self.h, C.byref(pp), float(inv_samples), out.ctypes.data_as(u8p),
+                                              w, h))
         return out
 
     def accum(self) -> tuple[np.ndarray, np.ndarray]:

@@ -28,6 +28,10 @@ LFS = """Renderer
 {
 	resolution 64 48
 	maxDepth 3
+	tonemapIndex %d
+	useVignette %d
+	vignetteIntensity 0.6
+	vignettePower 1.5
 }
 
 Camera
@@ -73,11 +77,23 @@ mesh
 """
 
 
+def _render_like_main(scene, ms):
+    sc = lfh.Scene().load(scene).build()
+    ctx = lfh.Context.from_scene(sc)
+    cam = sc.camera()
+    while True:
+        ctx.update(cam)
+        if ctx.sample_count() == ms + 1:
+            break
+        ctx.render()
+    return sc, ctx
+
+
 def test_cli_renders_scene_file_like_the_library(tmp_path):
     print("[cli] lfh_render", flush=True)
     (tmp_path / "floor.obj").write_text(OBJ_QUAD)
     scene = tmp_path / "scene.lfs"
-    scene.write_text(LFS)
+    scene.write_text(LFS % (0, 0))
     ms = 8
     r = subprocess.run([str(CLI), "-s", str(scene), "-ms", str(ms), "-ou", str(tmp_path), "-et", "pfm"],
                        capture_output=True, text=True, timeout=120)
@@ -87,14 +103,31 @@ def test_cli_renders_scene_file_like_the_library(tmp_path):
     assert data.startswith(header)
     img = np.frombuffer(data[len(header):], np.float32).reshape(48, 64, 3)
     assert np.isfinite(img).all() and img.mean() > 0
-    # the same file through the C ABI and the same Update / Render sequence
-    sc = lfh.Scene().load(scene).build()
-    ctx = lfh.Context.from_scene(sc)
-    cam = sc.camera()
-    while True:
-        ctx.update(cam)
-        if ctx.sample_count() == ms + 1:
-            break
-        ctx.render()
-    ref = ctx.output_hdr() * np.float32((ms + 1) / ms)
+    # the same file through the C ABI and the same Update / Render sequence:
+    # tonemap 0 exports accum / maxSamples
+    sc, ctx = _render_like_main(scene, ms)
+    s, c = ctx.accum()
+    ref = (s - c) * np.float32(1.0 / ms)
     np.testing.assert_array_equal(img, ref)
+
+
+def test_cli_ppm_export_is_the_post_processed_rgb8(tmp_path):
+    """-et ppm writes GetOutputBuffer's image: ACES tonemap and vignette from
+    the Renderer block, RGB8, rows top to bottom."""
+    print("[cli] lfh_render ppm", flush=True)
+    (tmp_path / "floor.obj").write_text(OBJ_QUAD)
+    scene = tmp_path / "scene.lfs"
+    scene.write_text(LFS % (2, 1))
+    ms = 6
+    r = subprocess.run([str(CLI), "-s", str(scene), "-ms", str(ms), "-ou", str(tmp_path), "-et", "ppm"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    data = (tmp_path / "scene.ppm").read_bytes()
+    header = b"P6\n64 48\n255\n"
+    assert data.startswith(header)
+    img = np.frombuffer(data[len(header):], np.uint8).reshape(48, 64, 3)[::-1]
+    sc, ctx = _render_like_main(scene, ms)
+    pp = sc.postprocess()
+    assert (pp.tonemap_index, pp.use_vignette) == (2, 1)
+    np.testing.assert_array_equal(img, ctx.output_ldr(pp, 1.0 / ms))
+    assert img.mean() > 5
