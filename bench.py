@@ -9,9 +9,12 @@ resident in HBM.  Prints one JSON line.
 
 Extra fields: any-hit throughput, coherent / incoherent split, traversal
 statistics, the roofline of the traversal kernel (algorithmic bytes per ray =
-32 + 24 + 64 N_int + 48 N_tri + 64 N_inst over the measured kernel time),
-and the CPU baseline (the oracle's FP32 restatement of ClosestHit on a
-bounded sample of the same rays, on all host cores).
+32 + 24 + 112 N_int + 48 N_tri + 64 N_inst over the measured kernel time:
+ray in, hit out, the six child-box rows and the child references of each
+BVH4 node visited, each triangle and instance record tested), against the
+nominal 8 TB/s and a measured device copy, and the CPU baseline (the oracle's
+FP32 restatement of ClosestHit on a bounded sample of the same rays, on all
+host cores).
 """
 from __future__ import annotations
 
@@ -29,6 +32,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+NODE_BYTES = 112   # BVH4 node bytes read per visit: 6 float4 of child boxes + 1 of child refs
 METRIC = "Mrays/s closest-hit; time-to-RMSE 1e-2 guided vs plain; p50/p99 job latency"
 
 
@@ -158,7 +162,7 @@ def main():
         ii = cnt[n_coh:].mean(0).tolist()
         extra["traversal_stats"] = {"N_int": round(n_int, 2), "N_tri": round(n_tri, 2), "N_inst": round(n_inst, 3),
                                     "coherent": [round(x, 2) for x in ci], "incoherent": [round(x, 2) for x in ii]}
-        bytes_per_ray = 32 + 24 + 64 * n_int + 48 * n_tri + 64 * n_inst
+        bytes_per_ray = 32 + 24 + NODE_BYTES * n_int + 48 * n_tri + 64 * n_inst
     else:
         bytes_per_ray = float("nan")
     achieved = bytes_per_ray * n / (ms_per_step * 1e-3) / 1e9
@@ -166,6 +170,12 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "bytes_per_ray": round(bytes_per_ray, 1)}
+    if not args.profile_only:
+        cp = copy_peak(stream)
+        roofline["copy_peak"] = round(cp, 1)
+        roofline["frac_of_copy_peak"] = round(achieved / cp, 4)
+        if traffic:
+            roofline["traffic_GBps"] = round(traffic / (ms_per_step * 1e-3) / 1e9, 1)
 
     cpu = None
     if not args.no_cpu_baseline and not args.profile_only:
@@ -202,6 +212,30 @@ def main():
     }
     out.update(extra)
     print(json.dumps(out), flush=True)
+
+
+def copy_peak(stream):
+    """Measured device-copy bandwidth (GB/s, read + write) of a 2 GiB buffer:
+    the practical HBM ceiling beside the nominal 8 TB/s."""
+    import torch
+    n = 1 << 29
+    x = torch.empty(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(2):
+        y.copy_(x)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    reps = 10
+    e0.record(stream)
+    for _ in range(reps):
+        y.copy_(x)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    gbs = 2 * 4 * n * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del x, y
+    torch.cuda.empty_cache()
+    log(f"device copy: {gbs:.1f} GB/s")
+    return gbs
 
 
 def read_traffic(path):
@@ -427,6 +461,37 @@ def _time_to(pts, target):
     return t * (e / target) ** 2, e > target
 
 
+GUIDE_FLOP_PER_QUERY = 2 * (3 * 64 * 64 + 64 * 40)   # 64-d encoding -> 3 x 64 ReLU -> 40 outputs
+
+
+def _guide_mlp_rate(guide, sc, stream, n=1 << 20, reps=10):
+    """MFMA rate of the guide's forward on n resident queries (points in the
+    scene box, random unit normals and directions), HIP events on the stream."""
+    import torch
+    g = torch.Generator(device="cpu").manual_seed(3)
+    v = sc.arrays()["vertices"][:, :3]
+    lo, hi = torch.tensor(v.min(0), dtype=torch.float32), torch.tensor(v.max(0), dtype=torch.float32)
+    pos = lo + (hi - lo) * torch.rand(n, 3, generator=g)
+    nrm = torch.nn.functional.normalize(torch.randn(n, 3, generator=g), dim=1)
+    wo = torch.nn.functional.normalize(torch.randn(n, 3, generator=g), dim=1)
+    x = torch.cat([pos, nrm, wo], 1).contiguous().cuda()
+    out = torch.empty(n, 40, dtype=torch.float32, device="cuda")
+    for _ in range(2):
+        guide.forward_device(x.data_ptr(), n, out.data_ptr(), stream.cuda_stream)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        guide.forward_device(x.data_ptr(), n, out.data_ptr(), stream.cuda_stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    tflops = GUIDE_FLOP_PER_QUERY * n / (ms * 1e-3) / 1e12
+    log(f"guide MLP forward: {n} queries in {ms:.3f} ms, {tflops:.1f} TFLOP/s")
+    return {"queries": n, "forward_ms": round(ms, 4), "flop_per_query": GUIDE_FLOP_PER_QUERY,
+            "tflops": round(tflops, 1), "frac_of_bf16_dense_peak": round(tflops / 2500.0, 4)}
+
+
 def guiding_leg(args):
     """Guiding room (about 200k triangles, light seen only through a door of 5 %
     of a wall), 1280 x 720, max depth 10.  Both runs use the consistent
@@ -482,6 +547,7 @@ def guiding_leg(args):
     guided = _converge(g_ctx, cam, ref, ref_var, stream, args.guide_max_spp, target)
     log(f"guided: {guided} (Adam steps {guide.steps()}, samples {g_ctx.guide_samples_seen()})")
     guided_rays = [round(r / (npix * guided[-1][0]), 3) for r in g_ctx.ray_counts()]
+    mlp = _guide_mlp_rate(guide, sc, stream)
     tp, xp = _time_to(plain, target)
     tg, xg = _time_to(guided, target)
     return {"metric": "wall time to relative RMSE 1e-2 (16x16 block means), guided / plain (consistent estimator)",
@@ -496,6 +562,7 @@ def guiding_leg(args):
                                 "guided": {"extended": guided_rays[0], "shadow": guided_rays[1]}},
             "ms_per_pass": {"plain": round(1e3 * plain[-1][1] / plain[-1][0], 3),
                             "guided": round(1e3 * guided[-1][1] / guided[-1][0], 3)},
+            "guide_mlp": mlp,
             "adam_steps": guide.steps(), "training_samples": g_ctx.guide_samples_seen(),
             "dtype": "f32 (MLP bf16/f32-acc)", "data": "synthetic",
             "config": {"workload": "guiding room ~200k triangles, 1280x720, depth 10, door 5% of wall",

@@ -338,6 +338,11 @@ int  lfh_guide_set_params(lfh_guide* g, const float* params, int64_t n_params);
 /* Forward on host arrays: inputs 9 floats/query (pos, normal, wo) -> 40 raw
  * outputs per query. */
 int  lfh_guide_forward(lfh_guide* g, const float* inputs9, int64_t n, float* out40);
+/* Forward on device arrays, asynchronous on `stream` (a hipStream_t; NULL:
+ * the null stream): inputs9 and out40 are device pointers of n x 9 and n x 40
+ * floats.  No reference counterpart: it serves callers whose queries are
+ * already resident, and timing of the MLP alone. */
+int  lfh_guide_forward_device(lfh_guide* g, const float* inputs9, int64_t n, float* out40, void* stream);
 /* Loss and gradients for a batch of (inputs, target direction, weight) samples:
  * loss = -sum_i w_i log p(d_i | x_i) / n; grads written as FP32 (n_params). */
 int  lfh_guide_loss_grad(lfh_guide* g, const float* inputs9, const float* dirs3,

@@ -21,10 +21,20 @@ using lfhd::kGuideLobeF4;
 using lfhd::kGuideOut;
 using lfhd::kGuideParams;
 
+#ifndef LFH_GUIDE_RECORD_LOG2
+#define LFH_GUIDE_RECORD_LOG2 18
+#endif
+#ifndef LFH_GUIDE_SAMPLE_BATCHES
+#define LFH_GUIDE_SAMPLE_BATCHES 4
+#endif
+#ifndef LFH_GUIDE_SERIAL_WARMUP
+#define LFH_GUIDE_SERIAL_WARMUP 1
+#endif
+
 namespace {
 constexpr int kBatch = 65536;             // samples per optimisation step
-constexpr int kMaxSamples = 4 * kBatch;   // samples kept from one training pass
-constexpr int kRecordedPaths = 1 << 18;   // target number of recorded paths per pass
+constexpr int kMaxSamples = LFH_GUIDE_SAMPLE_BATCHES * kBatch;  // samples kept from one wavefront
+constexpr int kRecordedPaths = 1 << LFH_GUIDE_RECORD_LOG2;     // recorded paths per wavefront, at most
 constexpr int kWarmSteps = 256;           // Adam steps taken at the warm-up rate
 constexpr int kWarmRepeats = 4;           // optimisation passes over a pass's samples while warming up
 constexpr float kWeightCap = 32.0f;       // online training: weights / their mean, capped
@@ -193,7 +203,7 @@ namespace lfh {
 // number afterwards.
 int guide_batch_limit(const lfh_ctx* c) {
     const lfh_guide* g = c->guide;
-    if (g && c->guide_train_every > 0 && g->steps < kWarmSteps) return 1;
+    if (LFH_GUIDE_SERIAL_WARMUP && g && c->guide_train_every > 0 && g->steps < kWarmSteps) return 1;
     return std::numeric_limits<int>::max();
 }
 
@@ -356,6 +366,15 @@ int lfh_guide_forward(lfh_guide* g, const float* inputs9, int64_t n, float* out4
     GCHK(g, hipMemcpyAsync(out40, g->out.p, (size_t)n * kGuideOut * sizeof(float), hipMemcpyDeviceToHost, g->stream),
          "lfh_guide_forward");
     GCHK(g, hipStreamSynchronize(g->stream), "lfh_guide_forward");
+    return LFH_OK;
+}
+
+int lfh_guide_forward_device(lfh_guide* g, const float* inputs9, int64_t n, float* out40, void* stream) {
+    int r = check_batch(g, "lfh_guide_forward_device", inputs9, out40, n);
+    if (r != LFH_OK || n == 0) return r;
+    GCHK(g, hipStreamSynchronize(g->stream), "lfh_guide_forward_device");  // parameter uploads
+    GCHK(g, lfhd::guide_forward_host(g->dev(), inputs9, (int)n, out40, nullptr, static_cast<hipStream_t>(stream)),
+         "lfh_guide_forward_device");
     return LFH_OK;
 }
 

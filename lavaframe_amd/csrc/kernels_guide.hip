@@ -125,6 +125,15 @@ LFH_DEV bool queue_query(const InferQueue& q, uint32_t i, V3& pos, V3& nrm, V3& 
     return true;
 }
 
+// Training record of a recorded path's vertex: the query inputs; k_shade adds
+// the sampled direction and the radiance terms.
+LFH_DEV void record_query(const GuideHook& gh, int path, int depth, V3 pos, V3 nrm, V3 wo) {
+    float4* r = gh.rec + ((size_t)(path / gh.stride) * gh.max_depth + depth) * 4;
+    r[0] = make_float4(pos.x, pos.y, pos.z, 0.0f);
+    r[1] = make_float4(nrm.x, nrm.y, nrm.z, 0.0f);
+    r[2] = make_float4(wo.x, wo.y, wo.z, -1.0f);
+}
+
 template <bool QUEUE>
 __global__ __launch_bounds__(kInferBlock) void k_guide_infer(GuideDev g, const float* __restrict__ in9, int n_host,
                                                              InferQueue q, float* __restrict__ raw40,
@@ -142,12 +151,7 @@ __global__ __launch_bounds__(kInferBlock) void k_guide_infer(GuideDev g, const f
             int path;
             const bool surf = queue_query(q, i, pos, nrm, wo, path);
             const GuideHook& gh = q.gh;
-            if (surf && gh.train && (path % gh.stride) == gh.phase) {
-                float4* r = gh.rec + ((size_t)(path / gh.stride) * gh.max_depth + q.depth) * 4;
-                r[0] = make_float4(pos.x, pos.y, pos.z, 0.0f);
-                r[1] = make_float4(nrm.x, nrm.y, nrm.z, 0.0f);
-                r[2] = make_float4(wo.x, wo.y, wo.z, -1.0f);
-            }
+            if (surf && gh.train && (path % gh.stride) == gh.phase) record_query(gh, path, q.depth, pos, nrm, wo);
         } else {
             const float* s = in9 + (size_t)i * 9;
             pos = v3(s[0], s[1], s[2]);

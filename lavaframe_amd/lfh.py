@@ -164,6 +164,7 @@ _SIGS = {
     "lfh_guide_get_params": (C.c_int, [_P, f32p, i64p]),
     "lfh_guide_set_params": (C.c_int, [_P, f32p, C.c_int64]),
     "lfh_guide_forward": (C.c_int, [_P, f32p, C.c_int64, f32p]),
+    "lfh_guide_forward_device": (C.c_int, [_P, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
     "lfh_guide_loss_grad": (C.c_int, [_P, f32p, f32p, f32p, C.c_int64, f32p, f32p]),
     "lfh_guide_train_step": (C.c_int, [_P, f32p, f32p, f32p, C.c_int64, f32p]),
     "lfh_set_guide": (C.c_int, [_P, _P, C.c_int32]),
@@ -671,6 +672,11 @@ class Guide:
         out = np.zeros((x.shape[0], 40), np.float32)
         self._chk(self.lib.lfh_guide_forward(self.h, fptr(x), x.shape[0], fptr(out)))
         return out
+
+    def forward_device(self, inputs9_ptr: int, n: int, out40_ptr: int, stream: int = 0):
+        """Forward on device arrays (n x 9 in, n x 40 out), asynchronous on stream."""
+        self._chk(self.lib.lfh_guide_forward_device(self.h, C.c_void_p(inputs9_ptr), n, C.c_void_p(out40_ptr),
+                                                    C.c_void_p(stream)))
 
     def sample(self, inputs9, seeds):
         x = np.ascontiguousarray(inputs9, np.float32)

@@ -287,3 +287,18 @@ def test_consistent_estimator_sees_lights_from_behind_one_sided_surfaces():
             assert np.isclose(g, c, rtol=1e-4, atol=1e-5).all(1).mean() > 0.99, (est, flipped)
     assert abs(means[1, True] - means[1, False]) < 0.03 * means[1, False], means
     assert means[0, True] < 0.5 * means[0, False], means
+
+
+def test_forward_device_equals_host_forward():
+    """lfh_guide_forward_device on resident queries gives the host entry's outputs bitwise."""
+    import torch
+    print("[guide] forward_device", flush=True)
+    g, _ = _guide(seed=5, scale=0.3)
+    n = 5000
+    x = _queries(n, 11)
+    host = g.forward(x)
+    dx = torch.from_numpy(x).cuda()
+    dout = torch.zeros(n, 40, dtype=torch.float32, device="cuda")
+    g.forward_device(dx.data_ptr(), n, dout.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(dout.cpu().numpy(), host)
