@@ -85,7 +85,16 @@ typedef struct lfh_hit {
  *   lights       15 floats / light {pos, emission, u, v, radius, area, type}
  *   hdr_*        env radiance RGB (3 floats/texel) + the two inverse-CDF tables
  *                (2 floats / entry), hdrloader.h:19-28.  hdr_width == 0: no env.
- * All pointers are borrowed for the duration of lfh_create only. */
+ * All pointers are borrowed for the duration of lfh_create only.
+ *
+ * Device BVH build (no reference counterpart): with nodes == NULL the BVHs are
+ * built on the GPU from the mesh and instance tables below, by lfh_create and
+ * at serving admission.  Mesh m owns the triangles [mesh_tri_offsets[m],
+ * mesh_tri_offsets[m + 1]) of vert_indices (ranges contiguous, in order, none
+ * empty); instance i draws mesh instance_meshes[i] with material
+ * instance_materials[i] and transforms[i].  Hits report the same prim ids as
+ * for the reference tree.  lfh_scene_get_desc fills these tables for built and
+ * flattened scenes alike. */
 typedef struct lfh_scene_desc {
     const float*   nodes;
     int64_t        num_nodes;
@@ -106,6 +115,10 @@ typedef struct lfh_scene_desc {
     const float*   hdr_cols;
     const float*   hdr_marginal;
     const float*   hdr_conditional;
+    const int64_t* mesh_tri_offsets;   /* num_meshes + 1 */
+    int32_t        num_meshes;
+    const int32_t* instance_meshes;    /* num_instances */
+    const int32_t* instance_materials; /* num_instances */
 } lfh_scene_desc;
 
 /* RenderOptions subset that reaches the path tracer (Renderer.h:18-67) and the
@@ -191,6 +204,10 @@ int  lfh_scene_get_camera(const lfh_scene* s, lfh_camera* cam);
 int  lfh_scene_get_postprocess(const lfh_scene* s, lfh_postprocess* pp);
 /* CreateAccelerationStructures (Scene.cpp:287-338). */
 int  lfh_scene_build(lfh_scene* s);
+/* The flattened arrays of CreateAccelerationStructures without its BVH builds
+ * (triangles in mesh order): lfh_scene_get_desc then returns nodes == NULL,
+ * for a device BVH build by lfh_create or lfh_server_submit. */
+int  lfh_scene_flatten(lfh_scene* s);
 /* RebuildInstances (Scene.cpp:272-285) after lfh_scene_set_instance_transform. */
 int  lfh_scene_set_instance_transform(lfh_scene* s, int32_t inst, const float* transform16);
 int  lfh_scene_rebuild_instances(lfh_scene* s);
@@ -211,7 +228,9 @@ int  lfh_synchronize(lfh_ctx* ctx);
 /* Renderer::Update's partial re-upload on instance edits (Renderer.cpp:192-203):
  * transforms (16 floats x num_instances), materials (28 floats x
  * num_materials) and TLAS nodes [first_node, first_node + count) (9 floats
- * each).  Any pointer may be NULL to leave that array unchanged. */
+ * each).  Any pointer may be NULL to leave that array unchanged.  A context
+ * whose BVHs were built on the device takes no TLAS nodes (tlas_nodes NULL):
+ * its TLAS is rebuilt on the device from the transforms. */
 int  lfh_update_instances(lfh_ctx* ctx, const float* transforms, const float* materials,
                           const float* tlas_nodes, int64_t first_node, int64_t count);
 /* TiledRenderer::Update: reset != 0 (camera moving / instances modified) clears

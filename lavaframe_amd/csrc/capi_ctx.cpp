@@ -68,7 +68,12 @@ int lfh_create(const lfh_scene_desc* desc, const lfh_options* opt, lfh_ctx** out
         return r;
     }
     c->stream = c->own_stream;
-    e = lfh::upload_scene(c->ps, &c->ds, c->stream);
+    std::string berr;
+    e = lfh::upload_scene(c->ps, &c->ds, c->stream, &berr);
+    if (!berr.empty()) {
+        lfh_destroy(c);
+        return ctx_fail(nullptr, LFH_E_INVALID, "lfh_create: device BVH build: " + berr);
+    }
     if (e == hipSuccess) {
         c->grid_blocks = lfh_trace_grid_blocks();
         size_t lanes = (size_t)c->grid_blocks * lfh_trace_threads_per_block();
@@ -117,6 +122,23 @@ int lfh_synchronize(lfh_ctx* c) {
 int lfh_update_instances(lfh_ctx* c, const float* transforms, const float* materials, const float* tlas_nodes,
                          int64_t first_node, int64_t count) {
     if (!c) return ctx_fail(nullptr, LFH_E_INVALID, "lfh_update_instances: NULL ctx");
+    if (c->ps.device_build) {
+        // no reference TLAS: the new transforms' TLAS is rebuilt on the device
+        if (tlas_nodes) return ctx_fail(c, LFH_E_INVALID, "lfh_update_instances: a device-built scene takes no TLAS nodes");
+        std::string err = lfh::repack_instances(&c->ps, transforms, nullptr, 0, 0, materials);
+        if (!err.empty()) return ctx_fail(c, LFH_E_INVALID, "lfh_update_instances: " + err);
+        HIPCHK(c, hipStreamSynchronize(c->stream), "lfh_update_instances");
+        err = lfh::device_build_tlas(&c->ps, &c->ds, c->stream);
+        if (!err.empty()) return ctx_fail(c, LFH_E_INVALID, "lfh_update_instances: " + err);
+        auto cpd = [&](DevBuf& b, const std::vector<float>& v) {
+            return hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(float), hipMemcpyHostToDevice, c->stream);
+        };
+        HIPCHK(c, cpd(c->ds.xforms, c->ps.xforms), "lfh_update_instances: transforms");
+        HIPCHK(c, cpd(c->ds.nmats, c->ps.nmats), "lfh_update_instances: normal matrices");
+        HIPCHK(c, cpd(c->ds.materials, c->ps.materials), "lfh_update_instances: materials");
+        HIPCHK(c, hipStreamSynchronize(c->stream), "lfh_update_instances");
+        return LFH_OK;
+    }
     std::string err = lfh::repack_instances(&c->ps, transforms, tlas_nodes, first_node, count, materials);
     if (!err.empty()) return ctx_fail(c, LFH_E_INVALID, "lfh_update_instances: " + err);
     HIPCHK(c, hipStreamSynchronize(c->stream), "lfh_update_instances");

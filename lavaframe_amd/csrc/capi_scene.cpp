@@ -184,6 +184,17 @@ int lfh_scene_build(lfh_scene* s) {
     return LFH_OK;
 }
 
+int lfh_scene_flatten(lfh_scene* s) {
+    if (!s) return fail_scene(nullptr, LFH_E_INVALID, "lfh_scene_flatten: NULL scene");
+    if (s->scene.meshInstances.empty()) return fail_scene(s, LFH_E_STATE, "lfh_scene_flatten: no instances");
+    for (const auto& mi : s->scene.meshInstances)
+        if (mi.materialID < 0 || mi.materialID >= (int)s->scene.materials.size())
+            return fail_scene(s, LFH_E_INVALID, "lfh_scene_flatten: instance material id out of range");
+    s->scene.Flatten();
+    s->built = true;
+    return LFH_OK;
+}
+
 int lfh_scene_set_instance_transform(lfh_scene* s, int32_t inst, const float* t16) {
     if (!s || !t16 || inst < 0 || inst >= (int)s->scene.meshInstances.size())
         return fail_scene(s, LFH_E_INVALID, "lfh_scene_set_instance_transform: bad instance");
@@ -202,9 +213,15 @@ int lfh_scene_get_desc(const lfh_scene* s, lfh_scene_desc* d) {
     if (!s->built) return fail_scene(nullptr, LFH_E_STATE, "lfh_scene_get_desc: scene not built");
     const Scene& sc = s->scene;
     std::memset(d, 0, sizeof *d);
-    d->nodes = reinterpret_cast<const float*>(sc.bvhTranslator.nodes.data());
-    d->num_nodes = (int64_t)sc.bvhTranslator.nodes.size();
-    d->top_level_index = sc.bvhTranslator.topLevelIndex;
+    if (!sc.flatOnly) {
+        d->nodes = reinterpret_cast<const float*>(sc.bvhTranslator.nodes.data());
+        d->num_nodes = (int64_t)sc.bvhTranslator.nodes.size();
+        d->top_level_index = sc.bvhTranslator.topLevelIndex;
+    }
+    d->mesh_tri_offsets = sc.meshTriOffsets.data();
+    d->num_meshes = (int32_t)sc.meshTriOffsets.size() - 1;
+    d->instance_meshes = sc.instanceMeshes.data();
+    d->instance_materials = sc.instanceMaterials.data();
     d->vert_indices = reinterpret_cast<const int32_t*>(sc.vertIndices.data());
     d->num_tris = (int64_t)sc.vertIndices.size();
     d->vertices_uvx = reinterpret_cast<const float*>(sc.verticesUVX.data());

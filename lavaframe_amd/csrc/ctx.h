@@ -62,6 +62,18 @@ struct PackedScene {
     std::vector<int32_t> flat_to_packed;
     std::vector<int32_t> flat_lrleaf;   // cached LRLeaf (3 ints / flat node)
     int32_t top_level_index = 0;
+    // Device BVH build (a desc without BVH nodes): inputs, then what the build
+    // leaves for instance updates.  nodes4 / tris / insts4 stay empty on the
+    // host; the build writes them on the device.
+    bool device_build = false;
+    std::vector<float> verts;           // 4 floats / vertex (x, y, z, u)
+    std::vector<int64_t> mesh_offsets;  // num_meshes + 1 triangle offsets
+    std::vector<int32_t> inst_mesh;     // mesh of each instance
+    std::vector<uint32_t> mesh_ref4;    // BVH4 ref of each mesh's BLAS
+    std::vector<float> mesh_box;        // 6 floats / mesh: BLAS root box
+    int blas_need = 0;                  // deepest BLAS stack need
+    int64_t blas_nodes4 = 0;            // BVH4 nodes of the BLAS part; the TLAS region follows
+    int64_t tlas_cap4 = 0;              // BVH4 nodes reserved for the TLAS
 };
 
 // Build the device layout from the reference flat arrays; "" on success.
@@ -72,6 +84,8 @@ std::string widen_bvh(PackedScene* ps);
 // Re-pack the instance-dependent parts after Renderer::Update-style edits.
 std::string repack_instances(PackedScene* ps, const float* transforms, const float* tlas_nodes,
                              int64_t first_node, int64_t count, const float* materials);
+// Instance records of a device-built scene: ps->insts with each instance's BLAS ref.
+void device_insts4(PackedScene* ps);
 
 struct DeviceScene {
     DevBuf nodes, tris, insts, xforms, nmats, vidx, normals, materials, lights, hdr_cols, hdr_marg, hdr_cond;
@@ -135,7 +149,13 @@ struct GuideBufs {
     int nslots = 0;                // record slots: recorded paths x max_depth, largest wavefront
 };
 
-hipError_t upload_scene(const PackedScene& ps, DeviceScene* ds, hipStream_t s);
+// Uploads the packed scene; for a desc without BVH nodes it also runs the
+// device BVH build (on failure *err, if given, says why).
+hipError_t upload_scene(PackedScene& ps, DeviceScene* ds, hipStream_t s, std::string* err = nullptr);
+// Device BVH build: BLAS of every mesh, then the TLAS; "" on success.
+std::string device_build(PackedScene* ps, DeviceScene* ds, hipStream_t s);
+// TLAS rebuild after instance edits of a device-built scene; "" on success.
+std::string device_build_tlas(PackedScene* ps, DeviceScene* ds, hipStream_t s);
 lfhd::TraceScene trace_view(const PackedScene& ps, const DeviceScene& ds);
 lfhd::ShadeScene shade_view(const PackedScene& ps, const DeviceScene& ds, const lfh_options& o);
 

@@ -780,6 +780,11 @@ void Scene::createBLAS() {
 }
 
 void Scene::RebuildInstances() {
+    if (flatOnly) {  // no host TLAS: a device-built scene rebuilds it from the transforms
+        for (size_t i = 0; i < meshInstances.size(); ++i) transforms[i] = meshInstances[i].transform;
+        instancesModified = true;
+        return;
+    }
     sceneBvh = Bvh(Bvh::kMedianSplit, 10.0f, 64);
     createTLAS();
     bvhTranslator.UpdateTLAS(&sceneBvh, meshInstances);
@@ -794,6 +799,7 @@ void Scene::CreateAccelerationStructures() {
     vertIndices.clear();
     verticesUVX.clear();
     normalsUVY.clear();
+    meshTriOffsets.assign(1, 0);
     int verticesCnt = 0;
     for (Mesh* m : meshes) {
         const std::vector<int>& tri = m->bvh.packed_indices();
@@ -803,9 +809,44 @@ void Scene::CreateAccelerationStructures() {
         verticesUVX.insert(verticesUVX.end(), m->verticesUVX.begin(), m->verticesUVX.end());
         normalsUVY.insert(normalsUVY.end(), m->normalsUVY.begin(), m->normalsUVY.end());
         verticesCnt += (int)m->verticesUVX.size();
+        meshTriOffsets.push_back((int64_t)vertIndices.size());
     }
     transforms.resize(meshInstances.size());
-    for (size_t i = 0; i < meshInstances.size(); ++i) transforms[i] = meshInstances[i].transform;
+    instanceMeshes.resize(meshInstances.size());
+    instanceMaterials.resize(meshInstances.size());
+    for (size_t i = 0; i < meshInstances.size(); ++i) {
+        transforms[i] = meshInstances[i].transform;
+        instanceMeshes[i] = meshInstances[i].meshID;
+        instanceMaterials[i] = meshInstances[i].materialID;
+    }
+    flatOnly = false;
+}
+
+void Scene::Flatten() {
+    vertIndices.clear();
+    verticesUVX.clear();
+    normalsUVY.clear();
+    meshTriOffsets.assign(1, 0);
+    int verticesCnt = 0;
+    for (Mesh* m : meshes) {
+        const int ntri = (int)(m->verticesUVX.size() / 3);
+        for (int index = 0; index < ntri; ++index)
+            vertIndices.push_back(Indices{index * 3 + 0 + verticesCnt, index * 3 + 1 + verticesCnt,
+                                          index * 3 + 2 + verticesCnt});
+        verticesUVX.insert(verticesUVX.end(), m->verticesUVX.begin(), m->verticesUVX.end());
+        normalsUVY.insert(normalsUVY.end(), m->normalsUVY.begin(), m->normalsUVY.end());
+        verticesCnt += (int)m->verticesUVX.size();
+        meshTriOffsets.push_back((int64_t)vertIndices.size());
+    }
+    transforms.resize(meshInstances.size());
+    instanceMeshes.resize(meshInstances.size());
+    instanceMaterials.resize(meshInstances.size());
+    for (size_t i = 0; i < meshInstances.size(); ++i) {
+        transforms[i] = meshInstances[i].transform;
+        instanceMeshes[i] = meshInstances[i].meshID;
+        instanceMaterials[i] = meshInstances[i].materialID;
+    }
+    flatOnly = true;
 }
 
 // ---------------------------------------------------------------------------

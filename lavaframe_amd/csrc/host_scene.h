@@ -298,6 +298,8 @@ public:
     void AddHDR(const float* rgb, int width, int height);
 
     void CreateAccelerationStructures();
+    // The flattened arrays without BVHs: every mesh's triangles in their own order.
+    void Flatten();
     void RebuildInstances();
 
     RenderOptions renderOptions;
@@ -306,6 +308,9 @@ public:
     std::vector<Vec4> verticesUVX;
     std::vector<Vec4> normalsUVY;
     std::vector<Mat4> transforms;
+    std::vector<int64_t> meshTriOffsets;     // triangle range of each mesh in vertIndices
+    std::vector<int32_t> instanceMeshes, instanceMaterials;
+    bool flatOnly = false;                   // Flatten(): no BVH arrays
     std::vector<Material> materials;
     std::vector<MeshInstance> meshInstances;
     bool instancesModified = false;

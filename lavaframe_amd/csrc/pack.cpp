@@ -1,5 +1,6 @@
 // Reference flat layout -> device layout (Renderer::Init's upload step,
 // Renderer.cpp:76-188, re-designed for gfx950 loads).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <unordered_map>
@@ -194,16 +195,62 @@ struct Builder {
         if (!ref_of(d.top_level_index, &ps->root)) return false;
         ps->top_level_index = d.top_level_index;
         ps->flat_lrleaf = lr;
+        return fill_rest(true);
+    }
 
+    // Scene arrays without a BVH: validated here, BVHs built on the device.
+    bool run_device() {
+        if (d.num_tris <= 0 || !d.vert_indices || !d.vertices_uvx) { err = "missing triangle arrays"; return false; }
+        if (d.num_tris >= (1 << 28)) { err = "too many triangles for a device build (2^28)"; return false; }
+        if (d.num_meshes <= 0 || d.num_meshes > 65535 || !d.mesh_tri_offsets) {
+            err = "a device build needs 1..65535 meshes with triangle offsets";
+            return false;
+        }
+        if (d.mesh_tri_offsets[0] != 0 || d.mesh_tri_offsets[d.num_meshes] != d.num_tris) {
+            err = "mesh triangle offsets must start at 0 and end at num_tris";
+            return false;
+        }
+        for (int m = 0; m < d.num_meshes; ++m)
+            if (d.mesh_tri_offsets[m + 1] <= d.mesh_tri_offsets[m]) { err = "empty mesh or decreasing offsets"; return false; }
+        if (d.num_instances <= 0 || !d.transforms || !d.instance_meshes || !d.instance_materials) {
+            err = "a device build needs instances with mesh and material ids";
+            return false;
+        }
+        if (d.num_materials <= 0 || !d.materials) { err = "at least one material is required"; return false; }
+        for (int64_t v = 0; v < d.num_vertices; ++v)
+            for (int a = 0; a < 3; ++a)
+                if (!std::isfinite(d.vertices_uvx[v * 4 + a])) { err = "non-finite vertex position"; return false; }
+        inst_blas.assign(d.num_instances, kRefSentinel);
+        inst_mat.assign(d.num_instances, 0);
+        ps->inst_mesh.resize(d.num_instances);
+        for (int i = 0; i < d.num_instances; ++i) {
+            const int m = d.instance_meshes[i];
+            if (m < 0 || m >= d.num_meshes) { err = "instance mesh out of range"; return false; }
+            ps->inst_mesh[i] = m;
+            inst_mat[i] = d.instance_materials[i];
+        }
+        ps->device_build = true;
+        ps->mesh_offsets.assign(d.mesh_tri_offsets, d.mesh_tri_offsets + d.num_meshes + 1);
+        ps->verts.assign(d.vertices_uvx, d.vertices_uvx + (size_t)d.num_vertices * 4);
+        return fill_rest(false);
+    }
+
+    // Triangles (records only for a host-built BVH), vertices, instances,
+    // materials, lights, environment.
+    bool fill_rest(bool tris) {
         // triangles
         ps->num_tris = d.num_tris;
         ps->num_vertices = d.num_vertices;
-        ps->tris.assign((size_t)d.num_tris * 12, 0.f);
+        ps->tris.assign(tris ? (size_t)d.num_tris * 12 : 0, 0.f);
         ps->vidx.assign((size_t)d.num_tris * 4, 0);
         for (int64_t k = 0; k < d.num_tris; ++k) {
             int32_t vi[3] = {d.vert_indices[k * 3], d.vert_indices[k * 3 + 1], d.vert_indices[k * 3 + 2]};
             for (int j = 0; j < 3; ++j)
                 if (vi[j] < 0 || vi[j] >= d.num_vertices) { err = "vertex index out of range"; return false; }
+            ps->vidx[k * 4 + 0] = vi[0];
+            ps->vidx[k * 4 + 1] = vi[1];
+            ps->vidx[k * 4 + 2] = vi[2];
+            if (!tris) continue;
             const float* v0 = &d.vertices_uvx[(size_t)vi[0] * 4];
             const float* v1 = &d.vertices_uvx[(size_t)vi[1] * 4];
             const float* v2 = &d.vertices_uvx[(size_t)vi[2] * 4];
@@ -211,9 +258,6 @@ struct Builder {
             t[0] = v0[0]; t[1] = v0[1]; t[2] = v0[2]; t[3] = u2f((uint32_t)(vi[0] / 3));
             t[4] = v1[0] - v0[0]; t[5] = v1[1] - v0[1]; t[6] = v1[2] - v0[2]; t[7] = u2f((uint32_t)k);
             t[8] = v2[0] - v0[0]; t[9] = v2[1] - v0[1]; t[10] = v2[2] - v0[2]; t[11] = 0.f;
-            ps->vidx[k * 4 + 0] = vi[0];
-            ps->vidx[k * 4 + 1] = vi[1];
-            ps->vidx[k * 4 + 2] = vi[2];
         }
         ps->normals.assign((size_t)d.num_vertices * 4, 0.f);
         if (d.normals_uvy && d.num_vertices > 0)
@@ -390,8 +434,20 @@ std::string widen_bvh(PackedScene* ps) {
 
 std::string pack_scene(const lfh_scene_desc& d, PackedScene* out) {
     Builder b(d, out);
+    if (!d.nodes) {
+        if (!b.run_device()) return b.err.empty() ? std::string("invalid scene") : b.err;
+        return std::string();
+    }
     if (!b.run()) return b.err.empty() ? std::string("invalid scene") : b.err;
     return widen_bvh(out);
+}
+
+void device_insts4(PackedScene* ps) {
+    ps->insts4 = ps->insts;
+    for (int i = 0; i < ps->num_instances; ++i) {
+        const uint32_t r = ps->mesh_ref4[ps->inst_mesh[i]];
+        std::memcpy(&ps->insts4[(size_t)i * 16 + 12], &r, 4);
+    }
 }
 
 // Renderer::Update re-uploads transforms, materials and the TLAS node range
@@ -413,15 +469,49 @@ std::string repack_instances(PackedScene* ps, const float* transforms, const flo
     if (tlas_nodes && count > 0) {
         const int64_t N = (int64_t)ps->flat_to_packed.size();
         if (first_node < 0 || first_node + count > N) return "TLAS node range out of bounds";
-        // Updated LRLeaf + boxes of the flat nodes in range.
-        std::vector<float> flat((size_t)N * 9, 0.f);
-        // Only the in-range nodes are needed; rebuild packed TLAS nodes.
+        // RebuildInstances may change the TLAS's shape (which flat nodes are
+        // internal) but not its size: the packed slots of its internal nodes
+        // are handed to the new internal nodes in flat order.
+        std::vector<int32_t> slots;
+        for (int64_t i = 0; i < count; ++i) {
+            const int64_t n = first_node + i;
+            if (ps->flat_lrleaf[n * 3 + 2] == 0 && ps->flat_to_packed[n] >= 0) slots.push_back(ps->flat_to_packed[n]);
+            ps->flat_to_packed[n] = -1;
+        }
+        std::sort(slots.begin(), slots.end());
         for (int64_t i = 0; i < count; ++i)
             for (int k = 0; k < 3; ++k) ps->flat_lrleaf[(first_node + i) * 3 + k] = (int32_t)tlas_nodes[i * 9 + 6 + k];
+        // only nodes reachable from the top level are packed (the spare
+        // nodes of the TLAS area are zero and look internal)
+        std::vector<uint8_t> reach((size_t)count, 0);
+        std::vector<int64_t> st{ps->top_level_index};
+        while (!st.empty()) {
+            const int64_t n = st.back();
+            st.pop_back();
+            if (n < first_node || n >= first_node + count || reach[n - first_node]) continue;
+            reach[n - first_node] = 1;
+            if (ps->flat_lrleaf[n * 3 + 2] == 0) {
+                st.push_back(ps->flat_lrleaf[n * 3]);
+                st.push_back(ps->flat_lrleaf[n * 3 + 1]);
+            }
+        }
+        size_t used = 0;
+        for (int64_t i = 0; i < count; ++i) {
+            const int64_t n = first_node + i;
+            if (!reach[i] || ps->flat_lrleaf[n * 3 + 2] != 0) continue;
+            if (used >= slots.size()) return "TLAS update adds internal nodes";
+            ps->flat_to_packed[n] = slots[used++];
+        }
+        if (used != slots.size()) return "TLAS update removes internal nodes";
+        if (ps->top_level_index >= first_node && ps->top_level_index < first_node + count) {
+            const int32_t lf = ps->flat_lrleaf[(int64_t)ps->top_level_index * 3 + 2];
+            ps->root = lf < 0 ? lfhd::make_inst((uint32_t)(-(int64_t)lf - 1))
+                              : (uint32_t)ps->flat_to_packed[ps->top_level_index];
+        }
         for (int64_t i = 0; i < count; ++i) {
             int64_t n = first_node + i;
             int32_t leaf = ps->flat_lrleaf[n * 3 + 2];
-            if (leaf < 0) {
+            if (reach[i] && leaf < 0) {
                 int64_t inst = -(int64_t)leaf - 1;
                 if (inst >= ps->num_instances) return "TLAS leaf instance out of range";
                 mat[inst] = ps->flat_lrleaf[n * 3 + 1];
@@ -436,7 +526,7 @@ std::string repack_instances(PackedScene* ps, const float* transforms, const flo
         };
         for (int64_t i = 0; i < count; ++i) {
             int64_t n = first_node + i;
-            if (ps->flat_lrleaf[n * 3 + 2] != 0) continue;
+            if (!reach[i] || ps->flat_lrleaf[n * 3 + 2] != 0) continue;
             int32_t p = ps->flat_to_packed[n];
             if (p < 0) return "TLAS update changes the node structure";
             int64_t L = ps->flat_lrleaf[n * 3], R = ps->flat_lrleaf[n * 3 + 1];
@@ -466,6 +556,7 @@ std::string repack_instances(PackedScene* ps, const float* transforms, const flo
             uint32_t m = (uint32_t)mat[i];
             std::memcpy(&ps->insts[(size_t)i * 16 + 13], &m, 4);
         }
+    if (ps->device_build) return std::string();  // the caller rebuilds the TLAS on the device
     const size_t n4 = ps->nodes4.size();
     std::string err = widen_bvh(ps);
     if (!err.empty()) return err;
@@ -481,11 +572,13 @@ static hipError_t up(DevBuf& b, const std::vector<T>& v, hipStream_t s) {
     return hipSuccess;
 }
 
-hipError_t upload_scene(const PackedScene& ps, DeviceScene* ds, hipStream_t s) {
+hipError_t upload_scene(PackedScene& ps, DeviceScene* ds, hipStream_t s, std::string* err) {
     hipError_t e;
-    if ((e = up(ds->nodes, ps.nodes4, s)) != hipSuccess) return e;
-    if ((e = up(ds->tris, ps.tris, s)) != hipSuccess) return e;
-    if ((e = up(ds->insts, ps.insts4, s)) != hipSuccess) return e;
+    if (!ps.device_build) {
+        if ((e = up(ds->nodes, ps.nodes4, s)) != hipSuccess) return e;
+        if ((e = up(ds->tris, ps.tris, s)) != hipSuccess) return e;
+        if ((e = up(ds->insts, ps.insts4, s)) != hipSuccess) return e;
+    }
     if ((e = up(ds->xforms, ps.xforms, s)) != hipSuccess) return e;
     if ((e = up(ds->nmats, ps.nmats, s)) != hipSuccess) return e;
     if ((e = up(ds->vidx, ps.vidx, s)) != hipSuccess) return e;
@@ -495,6 +588,13 @@ hipError_t upload_scene(const PackedScene& ps, DeviceScene* ds, hipStream_t s) {
     if ((e = up(ds->hdr_cols, ps.hdr_cols, s)) != hipSuccess) return e;
     if ((e = up(ds->hdr_marg, ps.hdr_marg, s)) != hipSuccess) return e;
     if ((e = up(ds->hdr_cond, ps.hdr_cond, s)) != hipSuccess) return e;
+    if (ps.device_build) {
+        std::string m = device_build(&ps, ds, s);
+        if (!m.empty()) {
+            if (err) *err = m;
+            return hipErrorInvalidValue;
+        }
+    }
     return hipStreamSynchronize(s);
 }
 

@@ -65,14 +65,22 @@ uint64_t hash_desc(const lfh_scene_desc& d) {
         h = fnv(h, d.hdr_marginal, (size_t)d.hdr_height * 8);
         h = fnv(h, d.hdr_conditional, n * 8);
     }
+    if (!d.nodes && d.num_meshes > 0) {  // device BVH build: the mesh and instance tables
+        h = fnv(h, d.mesh_tri_offsets, (size_t)(d.num_meshes + 1) * 8);
+        h = fnv(h, d.instance_meshes, (size_t)d.num_instances * 4);
+        h = fnv(h, d.instance_materials, (size_t)d.num_instances * 4);
+    }
     return h;
 }
 
 int64_t packed_bytes(const PackedScene& ps) {
-    return (int64_t)(ps.nodes.size() + ps.tris.size() + ps.insts.size() + ps.xforms.size() + ps.nmats.size() +
-                     ps.normals.size() + ps.materials.size() + ps.lights.size() + ps.hdr_cols.size() +
-                     ps.hdr_marg.size() + ps.hdr_cond.size()) * 4 +
-           (int64_t)ps.vidx.size() * 4;
+    int64_t b = (int64_t)(ps.nodes.size() + ps.tris.size() + ps.insts.size() + ps.xforms.size() + ps.nmats.size() +
+                          ps.normals.size() + ps.materials.size() + ps.lights.size() + ps.hdr_cols.size() +
+                          ps.hdr_marg.size() + ps.hdr_cond.size()) * 4 +
+                (int64_t)ps.vidx.size() * 4;
+    if (ps.device_build)  // resident after the build: triangle records, BVH4 nodes (at most one per 3 triangles + TLAS)
+        b += ps.num_tris * 48 + (ps.num_tris / 3 + 2 * (int64_t)ps.num_instances + 2) * 128;
+    return b;
 }
 
 struct SceneEntry {
@@ -265,7 +273,10 @@ int lfh_server_step(lfh_server* s, int32_t* active) {
         if (s->used + need > s->budget) evict_for(s, need);
         if (s->used + need > s->budget) continue;
         if (!e.resident) {
-            SCHK(s, lfh::upload_scene(e.ps, &e.ds, s->stream), "lfh_server_step: scene upload");
+            std::string berr;
+            hipError_t ue = lfh::upload_scene(e.ps, &e.ds, s->stream, &berr);
+            if (!berr.empty()) return sfail(s, LFH_E_INVALID, "lfh_server_step: device BVH build: " + berr);
+            SCHK(s, ue, "lfh_server_step: scene upload");
             e.resident = true;
             s->used += e.bytes;
         }

@@ -55,6 +55,8 @@ class SceneDesc(C.Structure):
         ("lights", f32p), ("num_lights", C.c_int32),
         ("hdr_width", C.c_int32), ("hdr_height", C.c_int32),
         ("hdr_cols", f32p), ("hdr_marginal", f32p), ("hdr_conditional", f32p),
+        ("mesh_tri_offsets", i64p), ("num_meshes", C.c_int32),
+        ("instance_meshes", i32p), ("instance_materials", i32p),
     ]
 
 
@@ -126,6 +128,7 @@ _SIGS = {
     "lfh_scene_set_instance_transform": (C.c_int, [_P, C.c_int32, f32p]),
     "lfh_scene_rebuild_instances": (C.c_int, [_P]),
     "lfh_scene_get_desc": (C.c_int, [_P, C.POINTER(SceneDesc)]),
+    "lfh_scene_flatten": (C.c_int, [_P]),
     "lfh_scene_counts": (C.c_int, [_P, i32p, i32p]),
     "lfh_create": (C.c_int, [C.POINTER(SceneDesc), C.POINTER(Options), C.POINTER(_P)]),
     "lfh_destroy": (None, [_P]),
@@ -379,6 +382,20 @@ class Scene:
         self._chk(self.lib.lfh_scene_build(self.h))
         return self
 
+    def set_instance_transform(self, inst: int, transform16):
+        t = np.ascontiguousarray(transform16, np.float32).reshape(16)
+        self._chk(self.lib.lfh_scene_set_instance_transform(self.h, int(inst), fptr(t)))
+
+    def rebuild_instances(self):
+        """RebuildInstances: the host TLAS and the flattened transforms."""
+        self._chk(self.lib.lfh_scene_rebuild_instances(self.h))
+
+    def flatten(self):
+        """The flattened arrays without host BVH builds (lfh_scene_flatten):
+        contexts made from this scene build their BVHs on the device."""
+        self._chk(self.lib.lfh_scene_flatten(self.h))
+        return self
+
     def desc(self) -> SceneDesc:
         d = SceneDesc()
         self._chk(self.lib.lfh_scene_get_desc(self.h, C.byref(d)))
@@ -400,6 +417,9 @@ class Scene:
             "materials": a(d.materials, d.num_materials * 28, np.float32).reshape(-1, 28),
             "transforms": a(d.transforms, d.num_instances * 16, np.float32).reshape(-1, 16),
             "lights": a(d.lights, d.num_lights * 15, np.float32).reshape(-1, 15),
+            "mesh_tri_offsets": a(d.mesh_tri_offsets, d.num_meshes + 1 if d.num_meshes else 0, np.int64),
+            "instance_meshes": a(d.instance_meshes, d.num_instances, np.int32),
+            "instance_materials": a(d.instance_materials, d.num_instances, np.int32),
         }
         if d.hdr_width:
             n = d.hdr_width * d.hdr_height
@@ -420,13 +440,17 @@ def desc_from_arrays(arr: dict) -> tuple[SceneDesc, list]:
         return a.ctypes.data_as(ptype), a.shape[0]
     d = SceneDesc()
     d.nodes, d.num_nodes = p("nodes", np.float32, f32p)
-    d.top_level_index = int(arr["top_level_index"])
+    d.top_level_index = int(arr.get("top_level_index", 0))
     d.vert_indices, d.num_tris = p("vert_indices", np.int32, i32p)
     d.vertices_uvx, d.num_vertices = p("vertices", np.float32, f32p)
     d.normals_uvy, _ = p("normals", np.float32, f32p)
     d.materials, d.num_materials = p("materials", np.float32, f32p)
     d.transforms, d.num_instances = p("transforms", np.float32, f32p)
     d.lights, d.num_lights = p("lights", np.float32, f32p)
+    d.mesh_tri_offsets, n_off = p("mesh_tri_offsets", np.int64, i64p)
+    d.num_meshes = max(n_off - 1, 0)
+    d.instance_meshes, _ = p("instance_meshes", np.int32, i32p)
+    d.instance_materials, _ = p("instance_materials", np.int32, i32p)
     if "hdr_cols" in arr and arr["hdr_cols"] is not None:
         cols = np.ascontiguousarray(arr["hdr_cols"], np.float32)
         d.hdr_height, d.hdr_width = cols.shape[:2]
@@ -451,11 +475,17 @@ class Context:
         self.options = options
 
     @classmethod
-    def from_scene(cls, scene: "Scene", **overrides):
+    def from_scene(cls, scene: "Scene", device_bvh: bool = False, **overrides):
+        """Context over a built or flattened scene; device_bvh: ignore the
+        host BVH and build on the device (always so for a flattened scene)."""
         o = scene.options()
         for k, v in overrides.items():
             setattr(o, k, v)
-        return cls(scene.desc(), o, scene.lib)
+        d = scene.desc()
+        if device_bvh:
+            d.nodes = None
+            d.num_nodes = 0
+        return cls(d, o, scene.lib)
 
     def close(self):
         if getattr(self, "h", None):
@@ -503,6 +533,18 @@ class Context:
                                                           C.c_void_p(d_counts)))
 
     # -- renderer surface --------------------------------------------------
+    def update_instances(self, transforms=None, materials=None, tlas_nodes=None, first_node: int = 0):
+        """Renderer::Update's re-upload after instance edits: transforms (n x 16),
+        materials (m x 28), TLAS nodes from first_node (k x 9; None for a
+        device-built context, whose TLAS is rebuilt from the transforms)."""
+        def arr(x):
+            return None if x is None else np.ascontiguousarray(x, np.float32)
+        t, m, n = arr(transforms), arr(materials), arr(tlas_nodes)
+        cnt = 0 if n is None else n.reshape(-1, 9).shape[0]
+        self._chk(self.lib.lfh_update_instances(self.h, fptr(t) if t is not None else None,
+                                                fptr(m) if m is not None else None,
+                                                fptr(n) if n is not None else None, int(first_node), cnt))
+
     def update(self, cam: Camera, reset: bool = False):
         self._chk(self.lib.lfh_update(self.h, C.byref(cam), 1 if reset else 0))
 

@@ -68,8 +68,9 @@ def unit_cube_tris() -> np.ndarray:
     return np.array(tris, np.float32).reshape(-1, 3, 3)
 
 
-def cornell(width=256, height=256, max_depth=4, lib=None) -> lfh.Scene:
-    """The Cornell configuration (unit box, open at z = 0, camera in front)."""
+def cornell(width=256, height=256, max_depth=4, lib=None, build=True) -> lfh.Scene:
+    """The Cornell configuration (unit box, open at z = 0, camera in front).
+    build=False: flattened without host BVHs (device build by the context)."""
     s = lfh.Scene(lib)
     s.add_material(lfh.MATERIAL_DEFAULT)                         # 0 default
     white = s.add_material(lfh.material((0.73, 0.73, 0.73)))
@@ -95,7 +96,7 @@ def cornell(width=256, height=256, max_depth=4, lib=None) -> lfh.Scene:
     s.set_camera((0.5, 0.5, -1.35), (0.5, 0.5, 0.5), 40.0)
     s.set_options(width=width, height=height, tile_width=width, tile_height=height, max_depth=max_depth,
                   enable_rr=1, rr_depth=2, use_env_map=0, use_constant_bg=0, hdr_multiplier=1.0)
-    return s.build()
+    return s.build() if build else s.flatten()
 
 
 def soup_triangles(n, seed=0) -> np.ndarray:
@@ -118,12 +119,12 @@ def soup_triangles(n, seed=0) -> np.ndarray:
     return v.astype(np.float32)
 
 
-def soup(n, seed=0, lib=None) -> lfh.Scene:
+def soup(n, seed=0, lib=None, build=True) -> lfh.Scene:
     s = lfh.Scene(lib)
     s.add_material(lfh.MATERIAL_DEFAULT)
     m = s.add_mesh(soup_triangles(n, seed))
     s.add_instance(m, affine(), 0)
-    return s.build()
+    return s.build() if build else s.flatten()
 
 
 def camera_basis(eye, lookat):
@@ -252,7 +253,7 @@ def sky_hdr(width=2048, height=1024, sun_dir=(0.3, 0.8, 0.5), sun_power=2000.0) 
 
 
 def instanced(n_meshes=64, n_instances=1024, min_tris=1000, max_tris=200000, width=1920, height=1080,
-              max_depth=8, hdr_size=(2048, 1024), seed=0, lib=None, light_count=8) -> lfh.Scene:
+              max_depth=8, hdr_size=(2048, 1024), seed=0, lib=None, light_count=8, build=True) -> lfh.Scene:
     rng = np.random.default_rng(seed)
     s = lfh.Scene(lib)
     s.add_material(lfh.MATERIAL_DEFAULT)
@@ -287,7 +288,7 @@ def instanced(n_meshes=64, n_instances=1024, min_tris=1000, max_tris=200000, wid
     s.set_camera((0.0, 12.0, -55.0), (0.0, 3.0, 0.0), 50.0)
     s.set_options(width=width, height=height, tile_width=width, tile_height=height, max_depth=max_depth,
                   enable_rr=1, rr_depth=2, use_env_map=1 if hdr_size else 0, use_constant_bg=0, hdr_multiplier=1.0)
-    return s.build()
+    return s.build() if build else s.flatten()
 
 
 def aperture_room(width=1280, height=720, max_depth=10, target_tris=200000, seed=0, lib=None) -> lfh.Scene:

@@ -1,0 +1,172 @@
+"""Device BVH build (PLOC clustering + BVH4 collapse on the GPU) against the
+host-built reference tree and the brute-force oracle: closest hits (prim,
+instance, material ids and the bits of t, u, v) and any hits identical;
+path-traced frames bitwise identical to the host-built context's; instance
+updates that rebuild the TLAS on the device; serving admission with a device
+build.  Hits do not depend on the tree: the nearest triangle wins, equal
+distances go to the lowest (instance, prim)."""
+import numpy as np
+import pytest
+
+from lavaframe_amd import lfh, scenes
+from oracle import native
+
+pytestmark = pytest.mark.gpu
+
+
+def aimed_rays(scene, n, seed=0):
+    v = scene.arrays()["vertices"][:, :3].reshape(-1, 3, 3)
+    rng = np.random.default_rng(seed)
+    tgt = v[rng.integers(0, len(v), n)].mean(1)
+    r = scenes.incoherent_rays(n, seed=seed + 1)
+    d = tgt - r["org"]
+    r["dir"] = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    return r
+
+
+def _wide_rays(n, seed):
+    """Incoherent rays from origins spread over the instanced scene's extent."""
+    r = scenes.incoherent_rays(n, seed=seed)
+    r["org"] = (r["org"] * np.array([60.0, 11.5, 60.0], np.float32) + np.array([-30.0, 0.5, -30.0], np.float32))
+    return r
+
+
+def _compare(gh, oh):
+    assert gh.shape == oh.shape
+    for k in ("prim", "inst", "mat"):
+        bad = np.nonzero(gh[k] != oh[k])[0]
+        assert bad.size == 0, f"{k} differs on {bad.size} rays, first {bad[:5]}: {gh[bad[:3]]} vs {oh[bad[:3]]}"
+    for k in ("t", "u", "v"):
+        bad = np.nonzero(gh[k].view(np.uint32) != oh[k].view(np.uint32))[0]
+        assert bad.size == 0, f"{k} bits differ on {bad.size} rays: {gh[bad[:3]]} vs {oh[bad[:3]]}"
+
+
+@pytest.mark.parametrize("ntri", [1, 3, 17, 1000, 50000])
+def test_soup_device_bvh_matches_oracle(ntri):
+    print(f"[dbvh] soup {ntri}", flush=True)
+    sc = scenes.soup(ntri, seed=ntri + 1)
+    ctx = lfh.Context.from_scene(sc, device_bvh=True)
+    orc = native.OracleScene(sc.arrays())
+    rays = np.concatenate([scenes.incoherent_rays(20000, seed=4), aimed_rays(sc, 20000, seed=9)])
+    gh = ctx.trace_closest(rays)
+    _compare(gh, orc.trace_closest(rays, tie_mode=1))
+    assert (gh["prim"] >= 0).sum() > 1000
+    assert np.array_equal(ctx.trace_any(rays), orc.trace_any(rays))
+
+
+def test_duplicated_triangles_tie_to_the_lowest_prim():
+    """Every triangle three times (identical centroids and keys): the build
+    still terminates and equal distances still go to the lowest prim id."""
+    print("[dbvh] duplicates", flush=True)
+    t = scenes.soup_triangles(500, seed=5)
+    s = lfh.Scene()
+    s.add_material(lfh.MATERIAL_DEFAULT)
+    s.add_instance(s.add_mesh(np.concatenate([t, t, t])), scenes.affine(), 0)
+    s.build()
+    ctx = lfh.Context.from_scene(s, device_bvh=True)
+    rays = aimed_rays(s, 20000, seed=2)
+    gh = ctx.trace_closest(rays)
+    _compare(gh, native.OracleScene(s.arrays()).trace_closest(rays, tie_mode=1))
+    hit = gh["prim"] >= 0
+    assert hit.sum() > 15000 and (gh["prim"][hit] < 500).all()
+
+
+def test_instanced_scene_device_bvh_equals_host_bvh():
+    print("[dbvh] instanced", flush=True)
+    sc = scenes.instanced(n_meshes=8, n_instances=96, min_tris=50, max_tris=8000, width=64, height=64,
+                          hdr_size=None, seed=3)
+    host = lfh.Context.from_scene(sc)
+    dev = lfh.Context.from_scene(sc, device_bvh=True)
+    rays = scenes.pinhole_rays(256, 128, eye=(0.0, 12.0, -55.0), lookat=(0.0, 3.0, 0.0), fov_deg=50.0)
+    rays = np.concatenate([rays, _wide_rays(30000, seed=8)])
+    gh, hh = dev.trace_closest(rays), host.trace_closest(rays)
+    _compare(gh, hh)
+    assert len(np.unique(gh["inst"][gh["inst"] >= 0])) > 20
+    assert np.array_equal(dev.trace_any(rays), host.trace_any(rays))
+
+
+def test_flattened_scene_hits_equal_built_scene():
+    """A scene flattened without any host BVH (triangles in mesh order) hits
+    the same prims as the built scene (whose vert_indices are BVH-ordered)."""
+    print("[dbvh] flattened", flush=True)
+    built = scenes.cornell(32, 32)
+    flat = scenes.cornell(32, 32, build=False)
+    assert flat.desc().num_nodes == 0 and not flat.desc().nodes
+    a = lfh.Context.from_scene(built)
+    b = lfh.Context.from_scene(flat)
+    rays = np.concatenate([scenes.incoherent_rays(50000, seed=5),
+                           scenes.pinhole_rays(128, 128, eye=(0.5, 0.5, -1.35), lookat=(0.5, 0.5, 0.5))])
+    ga, gb = a.trace_closest(rays), b.trace_closest(rays)
+    for k in ("prim", "inst", "mat"):
+        assert np.array_equal(ga[k], gb[k]), k
+    for k in ("t", "u", "v"):
+        assert np.array_equal(ga[k].view(np.uint32), gb[k].view(np.uint32)), k
+
+
+@pytest.mark.parametrize("scene", ["cornell", "instanced"])
+def test_path_traced_frames_bitwise_equal(scene):
+    print(f"[dbvh] frames {scene}", flush=True)
+    if scene == "cornell":
+        sc = scenes.cornell(48, 40)
+    else:
+        sc = scenes.instanced(n_meshes=6, n_instances=40, min_tris=50, max_tris=4000, width=48, height=40,
+                              hdr_size=(64, 32), seed=1)
+    outs = []
+    for dev in (False, True):
+        ctx = lfh.Context.from_scene(sc, device_bvh=dev)
+        ctx.update(sc.camera(), reset=True)
+        ctx.render_passes(6)
+        s, c = ctx.accum()
+        outs.append((s, c))
+    assert np.array_equal(outs[0][0].view(np.uint32), outs[1][0].view(np.uint32))
+    assert np.array_equal(outs[0][1].view(np.uint32), outs[1][1].view(np.uint32))
+    assert outs[0][0][..., :3].sum() > 0
+
+
+def test_instance_update_rebuilds_the_device_tlas():
+    """Moving instances: the device-built context (TLAS rebuilt on the
+    device) hits what a host-built context of the edited scene hits."""
+    print("[dbvh] instance update", flush=True)
+    sc = scenes.instanced(n_meshes=5, n_instances=30, min_tris=50, max_tris=3000, width=32, height=32,
+                          hdr_size=None, seed=6)
+    dev = lfh.Context.from_scene(sc, device_bvh=True)
+    rng = np.random.default_rng(1)
+    arr = sc.arrays()
+    T = arr["transforms"].copy()
+    for i in rng.choice(30, 10, replace=False):
+        t = T[i].reshape(4, 4).copy()
+        t[3, :3] += rng.uniform(-8, 8, 3).astype(np.float32)   # column-major: row 3 = translation
+        sc.set_instance_transform(int(i), t.reshape(16))
+    sc.rebuild_instances()
+    T2 = sc.arrays()["transforms"]
+    assert not np.array_equal(T, T2)
+    dev.update_instances(transforms=T2)
+    ref = lfh.Context.from_scene(sc)          # host-built from the edited scene
+    rays = np.concatenate([scenes.pinhole_rays(256, 128, eye=(0.0, 12.0, -55.0), lookat=(0.0, 3.0, 0.0),
+                                               fov_deg=50.0), _wide_rays(20000, seed=3)])
+    _compare(dev.trace_closest(rays), ref.trace_closest(rays))
+    # and the host-built context updated the reference way agrees too
+    host = lfh.Context.from_scene(scenes.instanced(n_meshes=5, n_instances=30, min_tris=50, max_tris=3000,
+                                                   width=32, height=32, hdr_size=None, seed=6))
+    a2 = sc.arrays()
+    top = int(a2["top_level_index"])
+    host.update_instances(transforms=T2, tlas_nodes=a2["nodes"][top:], first_node=top)
+    _compare(host.trace_closest(rays), ref.trace_closest(rays))
+
+
+def test_serving_admits_a_device_built_scene():
+    """The same job from a desc without BVH nodes (built at admission) and
+    from the host-built desc, in one server: bitwise equal images."""
+    print("[dbvh] serving", flush=True)
+    sc = scenes.cornell(40, 32)
+    dev_desc = sc.desc()
+    dev_desc.nodes = None
+    dev_desc.num_nodes = 0
+    srv = lfh.Server(max_jobs=4)
+    a = srv.submit(dev_desc, sc.options(), sc.camera(), target_spp=5)
+    b = srv.submit_scene(sc, 5)
+    srv.run_until_idle()
+    ia, ib = srv.result(a, 40, 32), srv.result(b, 40, 32)
+    assert srv.status(a).state == 2 and srv.status(b).state == 2
+    assert np.array_equal(ia.view(np.uint32), ib.view(np.uint32))
+    assert ia.sum() > 0
