@@ -36,9 +36,17 @@
 namespace lfhd {
 namespace {
 
+#ifndef LFH_PLOC_RADIUS
+#define LFH_PLOC_RADIUS 8
+#endif
+#ifndef LFH_LEAF_TRIS
+#define LFH_LEAF_TRIS 1
+#endif
+// On the 1M-triangle soup, radius 8 with one triangle per leaf traces fastest
+// of radius 4-32 and leaves of 1-4 triangles.
 constexpr int kBlk = 256;
-constexpr int kRadius = 16;     // PLOC search radius
-constexpr int kLeafTris = 4;    // largest BLAS leaf (the node ref holds up to 7)
+constexpr int kRadius = LFH_PLOC_RADIUS;   // PLOC search radius
+constexpr int kLeafTris = LFH_LEAF_TRIS;   // largest BLAS leaf (the node ref holds up to 7)
 
 LFH_DEV uint32_t f2ord(float f) {
     const uint32_t u = __float_as_uint(f);
@@ -61,6 +69,13 @@ LFH_DEV uint64_t spread16(uint32_t x) {
     v = (v | (v << 2)) & 0x1249249249249249ull;
     return v;
 }
+
+// Boxes written into BVH4 nodes are one float step wider on every side: the
+// triangle test uses v0 + (v1 - v0) rounded, which may lie half an ulp outside
+// the box of the original vertices.  One step on every box keeps each box
+// inside its parent's.  (Keys and clustering use the exact boxes.)
+LFH_DEV float down(float x) { return nextafterf(x, -__builtin_inff()); }
+LFH_DEV float up(float x) { return nextafterf(x, __builtin_inff()); }
 
 LFH_DEV float half_area(float4 lo, float4 hi) {
     const float dx = hi.x - lo.x, dy = hi.y - lo.y, dz = hi.z - lo.z;
@@ -426,8 +441,9 @@ __global__ __launch_bounds__(kBlk) void k_level_write(CollapseArgs a, Level q, i
         float4 lo = make_float4(1.f, 1.f, 1.f, 0.f), hi = make_float4(-1.f, -1.f, -1.f, 0.f);
         if (k < c) {
             const int n = ch[k];
-            lo = a.nbox[(size_t)n * 2];
-            hi = a.nbox[(size_t)n * 2 + 1];
+            const float4 l = a.nbox[(size_t)n * 2], h = a.nbox[(size_t)n * 2 + 1];
+            lo = make_float4(down(l.x), down(l.y), down(l.z), 0.f);
+            hi = make_float4(up(h.x), up(h.y), up(h.z), 0.f);
             if (is_internal(a, n) && !is_leaf_sized(a, n)) {
                 nx.n2[slot] = n;
                 nx.n4[slot] = next_base4 + slot;
@@ -460,8 +476,8 @@ __global__ __launch_bounds__(kBlk) void k_root_boxes(const int* __restrict__ roo
     if (m >= nseg) return;
     const float4 lo = nbox[(size_t)roots[m] * 2], hi = nbox[(size_t)roots[m] * 2 + 1];
     float* o = out6 + (size_t)m * 6;
-    o[0] = lo.x; o[1] = lo.y; o[2] = lo.z;
-    o[3] = hi.x; o[4] = hi.y; o[5] = hi.z;
+    o[0] = down(lo.x); o[1] = down(lo.y); o[2] = down(lo.z);
+    o[3] = up(hi.x); o[4] = up(hi.y); o[5] = up(hi.z);
 }
 
 inline int nblk(int64_t n) { return (int)((n + kBlk - 1) / kBlk); }

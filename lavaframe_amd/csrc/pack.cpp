@@ -299,17 +299,20 @@ struct Builder {
 namespace {
 // Collapses the packed BVH2 into a BVH4: each BVH4 node takes the two children
 // of a BVH2 node and keeps replacing its internal child of largest surface
-// area by that child's two children until it has four.  Every child keeps the
-// exact float box of its BVH2 node, and a BVH2 box contains its children's
+// area by that child's two children until it has four.  Every such child keeps
+// the exact float box of its BVH2 node, and a BVH2 box contains its children's
 // boxes, so a ray passes a collapsed child's slab test only if it passes the
-// tests of the BVH2 nodes skipped above it: the triangles tested (and the
-// hits found) are those of the binary traversal.  Nodes are allocated in
-// preorder, so a node's first child follows it in memory.
+// tests of the BVH2 nodes skipped above it.  A node left with room after that
+// (all its children leaves) splits its multi-triangle leaves into one child
+// per triangle, boxed by the triangle as the traversal tests it (v0, v0 + e0,
+// v0 + e1, one ulp outward): fewer triangle tests, the same closest hits.
+// Nodes are allocated in preorder, so a node's first child follows it in memory.
 struct Widen {
     const std::vector<float>& n2;
     std::vector<float>& out;
     std::unordered_map<uint32_t, std::pair<uint32_t, int>> memo;  // BLAS root -> (ref4, need)
     const std::vector<float>& insts;
+    const std::vector<float>& tris;   // 12 floats per triangle slot: v0, e0, e1
 
     struct Child {
         float box[6];
@@ -327,6 +330,14 @@ struct Widen {
         std::memcpy(&c[1].ref, o + 13, 4);
     }
     static bool internal(uint32_t r) { return r != lfhd::kRefSentinel && !lfhd::ref_is_leaf(r) && !lfhd::ref_is_inst(r); }
+    void tri_box(uint32_t slot, float* b) const {
+        const float* t = &tris[(size_t)slot * 12];
+        for (int a = 0; a < 3; ++a) {
+            const float p0 = t[a], p1 = t[a] + t[4 + a], p2 = t[a] + t[8 + a];
+            b[a] = std::nextafter(std::min(std::min(p0, p1), p2), -INFINITY);
+            b[3 + a] = std::nextafter(std::max(std::max(p0, p1), p2), INFINITY);
+        }
+    }
 
     // Returns the BVH4 ref of BVH2 ref r and, in *need, the largest number of
     // stack entries a traversal below it can hold.
@@ -360,6 +371,28 @@ struct Widen {
             ch[best + 1] = sub[1];
             ++n;
         }
+        while (n < 4) {
+            int best = -1;
+            float ba = -1.0f;
+            for (int k = 0; k < n; ++k) {
+                const uint32_t r = ch[k].ref;
+                if (r != lfhd::kRefSentinel && lfhd::ref_is_leaf(r) && lfhd::leaf_count(r) >= 2 &&
+                    n - 1 + (int)lfhd::leaf_count(r) <= 4 && area(ch[k].box) > ba) {
+                    ba = area(ch[k].box);
+                    best = k;
+                }
+            }
+            if (best < 0) break;
+            const uint32_t first = lfhd::leaf_first(ch[best].ref), cnt = lfhd::leaf_count(ch[best].ref);
+            Child sub[4];
+            for (uint32_t t = 0; t < cnt; ++t) {
+                sub[t].ref = lfhd::make_leaf(first + t, 1);
+                tri_box(first + t, sub[t].box);
+            }
+            for (int k = n - 1; k > best; --k) ch[k + (int)cnt - 1] = ch[k];
+            for (uint32_t t = 0; t < cnt; ++t) ch[best + (int)t] = sub[t];
+            n += (int)cnt - 1;
+        }
         const size_t idx = out.size() / 32;
         out.resize(out.size() + 32, 0.0f);
         uint32_t refs[4] = {lfhd::kRefSentinel, lfhd::kRefSentinel, lfhd::kRefSentinel, lfhd::kRefSentinel};
@@ -391,7 +424,7 @@ struct Widen {
 
 std::string widen_bvh(PackedScene* ps) {
     ps->nodes4.clear();
-    Widen w{ps->nodes, ps->nodes4, {}, ps->insts};
+    Widen w{ps->nodes, ps->nodes4, {}, ps->insts, ps->tris};
     // BLAS trees first, in instance order, then the TLAS
     for (int i = 0; i < ps->num_instances; ++i) {
         uint32_t blas;
