@@ -119,6 +119,12 @@ typedef struct lfh_scene_desc {
     int32_t        num_meshes;
     const int32_t* instance_meshes;    /* num_instances */
     const int32_t* instance_materials; /* num_instances */
+    /* Texture maps (Scene.h textureMapsArray / texWidth / texHeight): RGBA8,
+     * num_textures layers of tex_width x tex_height texels, first row first.
+     * A material's last four floats name its albedo, metallic-roughness,
+     * normal and emission layers (-1: none). */
+    const uint8_t* textures;
+    int32_t        tex_width, tex_height, num_textures;
 } lfh_scene_desc;
 
 /* RenderOptions subset that reaches the path tracer (Renderer.h:18-67) and the
@@ -187,6 +193,12 @@ int  lfh_scene_add_mesh_file(lfh_scene* s, const char* path, int32_t* id);
 int  lfh_scene_add_mesh(lfh_scene* s, const float* positions, const float* normals,
                         const float* uvs, int64_t num_tris, int32_t* id);
 int  lfh_scene_add_material(lfh_scene* s, const float* material28, int32_t* id);
+/* Scene::AddTexture (Scene.cpp:45-66) with decoded RGBA8 pixels (rows top to
+ * bottom, as stbi_load returns them); a name seen before returns its id. */
+int  lfh_scene_add_texture(lfh_scene* s, const uint8_t* rgba8, int32_t width, int32_t height,
+                           const char* name, int32_t* id);
+/* Scene::AddTexture from a file: PNG or binary PPM/PGM. */
+int  lfh_scene_add_texture_file(lfh_scene* s, const char* path, int32_t* id);
 int  lfh_scene_add_light(lfh_scene* s, const float* light15, int32_t* id);
 int  lfh_scene_add_instance(lfh_scene* s, int32_t mesh_id, const float* transform16,
                             int32_t material_id, int32_t* id);

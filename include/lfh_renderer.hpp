@@ -41,6 +41,10 @@ public:
     HipRendererT(const HipRendererT&) = delete;
     HipRendererT& operator=(const HipRendererT&) = delete;
 
+    // Build the BVHs on the device at Init (from the scene's meshes and
+    // instances) instead of uploading the scene's own BVH arrays.
+    void SetDeviceBvh(bool on) { device_bvh_ = on; }
+
     virtual void Init() {
         if (ctx_) return;
         if (!scene_) throw std::runtime_error("HipRenderer: no scene");
@@ -60,7 +64,13 @@ public:
     virtual void Update(float /*secondsElapsed*/) {
         if (!ctx_) return;
         bool reset = scene_->camera->isMoving || scene_->instancesModified;
-        if (scene_->instancesModified) {
+        if (scene_->instancesModified && device_bvh_) {
+            // transforms and materials; the TLAS is rebuilt on the device
+            check(lfh_update_instances(ctx_, reinterpret_cast<const float*>(scene_->transforms.data()),
+                                       reinterpret_cast<const float*>(scene_->materials.data()), nullptr, 0, 0),
+                  "Update");
+            scene_->instancesModified = false;
+        } else if (scene_->instancesModified) {
             // Renderer::Update: transforms, materials and the TLAS node range
             const int64_t top = scene_->bvhTranslator.topLevelIndex;
             const int64_t count = (int64_t)scene_->bvhTranslator.nodes.size() - top;
@@ -159,6 +169,29 @@ protected:
             d.hdr_marginal = reinterpret_cast<const float*>(hdr_ptr(s.hdrData->marginalDistData));
             d.hdr_conditional = reinterpret_cast<const float*>(hdr_ptr(s.hdrData->conditionalDistData));
         }
+        if (!s.textures.empty()) {   // textureMapsArray (Scene.cpp:223-230)
+            d.textures = reinterpret_cast<const uint8_t*>(s.textureMapsArray.data());
+            d.tex_width = s.texWidth;
+            d.tex_height = s.texHeight;
+            d.num_textures = (int32_t)s.textures.size();
+        }
+        if (device_bvh_) {
+            // mesh triangle ranges follow vertIndices' mesh order (Scene.cpp:198-214)
+            mesh_offsets_.assign(1, 0);
+            for (const auto* m : s.meshes) mesh_offsets_.push_back(mesh_offsets_.back() + (int64_t)(m->verticesUVX.size() / 3));
+            inst_mesh_.clear();
+            inst_mat_.clear();
+            for (const auto& mi : s.meshInstances) {
+                inst_mesh_.push_back(mi.meshID);
+                inst_mat_.push_back(mi.materialID);
+            }
+            d.nodes = nullptr;
+            d.num_nodes = 0;
+            d.mesh_tri_offsets = mesh_offsets_.data();
+            d.num_meshes = (int32_t)s.meshes.size();
+            d.instance_meshes = inst_mesh_.data();
+            d.instance_materials = inst_mat_.data();
+        }
         return d;
     }
 
@@ -209,6 +242,9 @@ protected:
     std::string shadersDirectory_;
     lfh_ctx* ctx_ = nullptr;
     int width_ = 0, height_ = 0;
+    bool device_bvh_ = false;
+    mutable std::vector<int64_t> mesh_offsets_;
+    mutable std::vector<int32_t> inst_mesh_, inst_mat_;
 };
 
 }  // namespace lfh

@@ -20,7 +20,7 @@
 int main(int argc, char** argv) {
     std::string scene_file, out_path = "./", name = "", type = "pfm";
     int max_samples = -1;
-    bool debug = false;
+    bool debug = false, device_bvh = false;
     for (int i = 1; i < argc; ++i) {
         std::string a = argv[i];
         auto next = [&](const char* what) -> std::string {
@@ -36,10 +36,12 @@ int main(int argc, char** argv) {
         else if (a == "-ep" || a == "-exportname") name = next("-ep");
         else if (a == "-et" || a == "--exporttype") type = next("-et");
         else if (a == "-db" || a == "--debug") debug = true;
+        else if (a == "-dbvh" || a == "--device-bvh") device_bvh = true;
         else if (a == "-u" || a == "--noui" || a == "-nm" || a == "--nomove" || a == "-w" || a == "--nowindow" ||
                  a == "-n" || a == "--neutral") {
         } else if (a == "-h" || a == "--help") {
-            std::printf("lfh_render -s scene.lfs -ms samples [-ou dir] [-ep name] [-et pfm|ppm] [-db]\n");
+            std::printf("lfh_render -s scene.lfs -ms samples [-ou dir] [-ep name] [-et pfm|ppm] [-db] [-dbvh]\n"
+                        "  -dbvh / --device-bvh: build the BVHs on the GPU instead of uploading the scene's\n");
             return 0;
         } else {
             std::fprintf(stderr, "unknown argument %s\n", a.c_str());
@@ -59,6 +61,7 @@ int main(int argc, char** argv) {
     }
     scene.camera->isMoving = false;
     lfh::HipRendererT<lfh::Scene> renderer(&scene);
+    renderer.SetDeviceBvh(device_bvh);
     try {
         renderer.Init();
         auto t0 = std::chrono::steady_clock::now();

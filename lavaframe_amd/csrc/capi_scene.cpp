@@ -37,7 +37,26 @@ int lfh_scene_load(lfh_scene* s, const char* path) {
     std::string err;
     if (!lfh::LoadSceneFromFile(path, &s->scene, s->scene.renderOptions, &err))
         return fail_scene(s, LFH_E_IO, "lfh_scene_load: " + err);
+    if (!s->scene.PackTextures()) return fail_scene(s, LFH_E_INVALID, "lfh_scene_load: " + s->scene.lastError);
     s->built = true;
+    return LFH_OK;
+}
+
+int lfh_scene_add_texture(lfh_scene* s, const uint8_t* rgba8, int32_t width, int32_t height, const char* name,
+                          int32_t* id) {
+    if (!s || !rgba8) return fail_scene(s, LFH_E_INVALID, "lfh_scene_add_texture: NULL argument");
+    if (width <= 0 || height <= 0 || width > 32768 || height > 32768)
+        return fail_scene(s, LFH_E_INVALID, "lfh_scene_add_texture: bad size");
+    int r = s->scene.AddTexture(rgba8, width, height, name ? std::string(name) : std::string());
+    if (id) *id = r;
+    return LFH_OK;
+}
+
+int lfh_scene_add_texture_file(lfh_scene* s, const char* path, int32_t* id) {
+    if (!s || !path) return fail_scene(s, LFH_E_INVALID, "lfh_scene_add_texture_file: NULL argument");
+    int r = s->scene.AddTexture(std::string(path));
+    if (id) *id = r;
+    if (r < 0) return fail_scene(s, LFH_E_IO, "lfh_scene_add_texture_file: " + s->scene.lastError);
     return LFH_OK;
 }
 
@@ -180,6 +199,7 @@ int lfh_scene_build(lfh_scene* s) {
         if (mi.materialID < 0 || mi.materialID >= (int)s->scene.materials.size())
             return fail_scene(s, LFH_E_INVALID, "lfh_scene_build: instance material id out of range");
     s->scene.CreateAccelerationStructures();
+    if (!s->scene.PackTextures()) return fail_scene(s, LFH_E_INVALID, "lfh_scene_build: " + s->scene.lastError);
     s->built = true;
     return LFH_OK;
 }
@@ -191,6 +211,7 @@ int lfh_scene_flatten(lfh_scene* s) {
         if (mi.materialID < 0 || mi.materialID >= (int)s->scene.materials.size())
             return fail_scene(s, LFH_E_INVALID, "lfh_scene_flatten: instance material id out of range");
     s->scene.Flatten();
+    if (!s->scene.PackTextures()) return fail_scene(s, LFH_E_INVALID, "lfh_scene_flatten: " + s->scene.lastError);
     s->built = true;
     return LFH_OK;
 }
@@ -222,6 +243,12 @@ int lfh_scene_get_desc(const lfh_scene* s, lfh_scene_desc* d) {
     d->num_meshes = (int32_t)sc.meshTriOffsets.size() - 1;
     d->instance_meshes = sc.instanceMeshes.data();
     d->instance_materials = sc.instanceMaterials.data();
+    if (!sc.textures.empty()) {
+        d->textures = sc.textureMapsArray.data();
+        d->tex_width = sc.texWidth;
+        d->tex_height = sc.texHeight;
+        d->num_textures = (int32_t)sc.textures.size();
+    }
     d->vert_indices = reinterpret_cast<const int32_t*>(sc.vertIndices.data());
     d->num_tris = (int64_t)sc.vertIndices.size();
     d->vertices_uvx = reinterpret_cast<const float*>(sc.verticesUVX.data());

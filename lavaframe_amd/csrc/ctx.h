@@ -54,6 +54,9 @@ struct PackedScene {
     std::vector<float> hdr_marg;  // 2 floats / row
     std::vector<float> hdr_cond;  // 2 floats / texel
     int hdr_w = 0, hdr_h = 0;
+    std::vector<uint8_t> tex;     // RGBA8 texture layers
+    int tex_w = 0, tex_h = 0, tex_layers = 0;
+    std::vector<float> uvs;       // 2 floats / vertex, only with textures
     uint32_t root = lfhd::kRefSentinel;
     int64_t num_internal = 0, num_tris = 0, num_vertices = 0;
     int32_t num_instances = 0, num_materials = 0, num_lights = 0;
@@ -88,15 +91,16 @@ std::string repack_instances(PackedScene* ps, const float* transforms, const flo
 void device_insts4(PackedScene* ps);
 
 struct DeviceScene {
-    DevBuf nodes, tris, insts, xforms, nmats, vidx, normals, materials, lights, hdr_cols, hdr_marg, hdr_cond;
+    DevBuf nodes, tris, insts, xforms, nmats, vidx, normals, materials, lights, hdr_cols, hdr_marg, hdr_cond, tex, uvs;
     void release() {
-        DevBuf* b[12] = {&nodes, &tris, &insts, &xforms, &nmats, &vidx, &normals, &materials, &lights,
-                         &hdr_cols, &hdr_marg, &hdr_cond};
+        DevBuf* b[14] = {&nodes, &tris, &insts, &xforms, &nmats, &vidx, &normals, &materials, &lights,
+                         &hdr_cols, &hdr_marg, &hdr_cond, &tex, &uvs};
         for (DevBuf* x : b) x->release();
     }
     size_t bytes() const {
         return nodes.bytes + tris.bytes + insts.bytes + xforms.bytes + nmats.bytes + vidx.bytes + normals.bytes +
-               materials.bytes + lights.bytes + hdr_cols.bytes + hdr_marg.bytes + hdr_cond.bytes;
+               materials.bytes + lights.bytes + hdr_cols.bytes + hdr_marg.bytes + hdr_cond.bytes + tex.bytes +
+               uvs.bytes;
     }
 };
 

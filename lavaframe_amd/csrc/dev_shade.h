@@ -78,6 +78,13 @@ LFH_DEV Mat load_material(const float4* mats, int id) {
     return m;
 }
 
+// Onb (pathtrace.glsl:7-13): tangent frame around N.
+LFH_DEV void onb(V3 N, V3& T, V3& B) {
+    V3 up = fabsf(N.z) < 0.999f ? v3(0.f, 0.f, 1.f) : v3(1.f, 0.f, 0.f);
+    T = normalize(cross(up, N));
+    B = cross(N, T);
+}
+
 struct Frame {
     V3 N;        // ffnormal (the BSDF's N)
     V3 T, B;     // tangent frame built from state.normal
@@ -397,6 +404,73 @@ LFH_DEV V3 tex_linear(const float4* cols, int W, int H, float s, float t) {
     V3 bot = c01 * (1.0f - a) + c11 * a;
     return top * (1.0f - b) + bot * b;
 }
+// ---- texture maps (GetMaterialsAndTextures, pathtrace.glsl:78-113) -------------------
+// GL_LINEAR read of RGBA8 layer `layer` (clamped to the array, as GL clamps the
+// layer coordinate) with GL_REPEAT wrapping, channels normalized to [0, 1].
+LFH_DEV float4 tex_rgba8(const ShadeScene& sc, float s, float t, int layer) {
+    const int W = sc.tex_w, H = sc.tex_h;
+    layer = layer < 0 ? 0 : (layer >= sc.tex_layers ? sc.tex_layers - 1 : layer);
+    float u = s * (float)W - 0.5f, v = t * (float)H - 0.5f;
+    float fu = floorf(u), fv = floorf(v);
+    float a = u - fu, b = v - fv;
+    // the integer texel index modulo the size, in float first (texture
+    // coordinates far outside [0, 1] repeat)
+    fu -= (float)W * floorf(fu / (float)W);
+    fv -= (float)H * floorf(fv / (float)H);
+    int i0 = wrapi((int)fu, W), i1 = wrapi((int)fu + 1, W);
+    int j0 = wrapi((int)fv, H), j1 = wrapi((int)fv + 1, H);
+    const uint32_t* L = sc.tex + (size_t)layer * W * H;
+    auto px = [&](int i, int j) {
+        const uint32_t c = L[(size_t)j * W + i];
+        const float k = 1.0f / 255.0f;
+        return make_float4((float)(c & 0xFFu) * k, (float)((c >> 8) & 0xFFu) * k, (float)((c >> 16) & 0xFFu) * k,
+                           (float)(c >> 24) * k);
+    };
+    const float4 c00 = px(i0, j0), c10 = px(i1, j0), c01 = px(i0, j1), c11 = px(i1, j1);
+    const float ia = 1.0f - a, ib = 1.0f - b;
+    const float4 top = make_float4(c00.x * ia + c10.x * a, c00.y * ia + c10.y * a, c00.z * ia + c10.z * a,
+                                   c00.w * ia + c10.w * a);
+    const float4 bot = make_float4(c01.x * ia + c11.x * a, c01.y * ia + c11.y * a, c01.z * ia + c11.z * a,
+                                   c01.w * ia + c11.w * a);
+    return make_float4(top.x * ib + bot.x * b, top.y * ib + bot.y * b, top.z * ib + bot.z * b,
+                       top.w * ib + bot.w * b);
+}
+
+LFH_DEV V3 pow22(float4 c) { return v3(powf(c.x, 2.2f), powf(c.y, 2.2f), powf(c.z, 2.2f)); }
+
+// The maps of material matId at texture coordinate tc (texUV.y = 1 - tc.y):
+// albedo *= map^2.2, metallic / roughness from the R / G channels (roughness
+// squared), the tangent-space normal map in the shading normal's frame, and
+// emission = map^2.2.
+LFH_DEV void apply_textures(const ShadeScene& sc, int matId, float2 tc, Mat& m, V3& nrm) {
+    if (!sc.tex) return;
+    const float4 ids = sc.materials[(size_t)matId * kMaterialF4 + 6];
+    const float s = tc.x, t = 1.0f - tc.y;
+    if ((int)ids.x >= 0) m.albedo = m.albedo * pow22(tex_rgba8(sc, s, t, (int)ids.x));
+    if ((int)ids.y >= 0) {
+        const float4 c = tex_rgba8(sc, s, t, (int)ids.y);
+        m.metallic = c.x;
+        m.roughness = fmaxf(c.y * c.y, 0.001f);
+    }
+    if ((int)ids.z >= 0) {
+        const float4 c = tex_rgba8(sc, s, t, (int)ids.z);
+        const V3 n = normalize(v3(c.x * 2.0f - 1.0f, c.y * 2.0f - 1.0f, c.z * 2.0f - 1.0f));
+        V3 T, B;
+        onb(nrm, T, B);
+        nrm = normalize(T * n.x + B * n.y + nrm * n.z);
+    }
+    if (ids.w >= 0.0f) m.emission = pow22(tex_rgba8(sc, s, t, (int)ids.w));
+}
+
+// Emission of material matId at texture coordinate tc (the emitter-hit case,
+// where only the emission of the stale material is used).
+LFH_DEV V3 textured_emission(const ShadeScene& sc, int matId, float2 tc, V3 emission) {
+    if (!sc.tex) return emission;
+    const float4 ids = sc.materials[(size_t)matId * kMaterialF4 + 6];
+    if (ids.w >= 0.0f) return pow22(tex_rgba8(sc, tc.x, 1.0f - tc.y, (int)ids.w));
+    return emission;
+}
+
 LFH_DEV float EnvPdf(const ShadeScene& sc, V3 d) {
     float theta = acosf(clampf(d.y, -1.0f, 1.0f));
     float u = (kPi + atan2f(d.z, d.x)) * (1.0f / kTwoPi);

@@ -1,6 +1,7 @@
 // Host-side scene surface: builders, flattening, OBJ ingest, HDR tables,
 // camera.  See host_scene.h for the reference types mirrored here.
 #include "host_scene.h"
+#include "host_image.h"
 
 #include <algorithm>
 #include <cmath>
@@ -723,6 +724,54 @@ int Scene::AddMesh(const float* positions, const float* normals, const float* uv
     return (int)meshes.size() - 1;
 }
 
+int Scene::AddTexture(const std::string& filename) {
+    for (size_t i = 0; i < textures.size(); ++i)
+        if (textures[i].name == filename) return (int)i;
+    Texture t;
+    Image img;
+    std::string e;
+    if (!load_image(filename, &img, &e)) {
+        lastError = e;
+        return -1;
+    }
+    t.name = filename;
+    t.width = img.width;
+    t.height = img.height;
+    t.rgba = std::move(img.rgba);
+    textures.push_back(std::move(t));
+    return (int)textures.size() - 1;
+}
+
+int Scene::AddTexture(const uint8_t* rgba, int width, int height, const std::string& name) {
+    if (!name.empty())
+        for (size_t i = 0; i < textures.size(); ++i)
+            if (textures[i].name == name) return (int)i;
+    Texture t;
+    t.name = name;
+    t.width = width;
+    t.height = height;
+    t.rgba.assign(rgba, rgba + (size_t)width * height * 4);
+    textures.push_back(std::move(t));
+    return (int)textures.size() - 1;
+}
+
+// Textures copied into one RGBA8 array of equally sized layers (Scene.cpp:223-230;
+// the GL array texture takes one width and height for every layer).
+bool Scene::PackTextures() {
+    textureMapsArray.clear();
+    texWidth = texHeight = 0;
+    for (const Texture& t : textures) {
+        if (texWidth && (t.width != texWidth || t.height != texHeight)) {
+            lastError = "all textures must have one size (one RGBA8 array texture holds them): " + t.name;
+            return false;
+        }
+        texWidth = t.width;
+        texHeight = t.height;
+        textureMapsArray.insert(textureMapsArray.end(), t.rgba.begin(), t.rgba.end());
+    }
+    return true;
+}
+
 int Scene::AddMaterial(const Material& m) {
     materials.push_back(m);
     return (int)materials.size() - 1;
@@ -908,9 +957,12 @@ bool LoadSceneFromFile(const std::string& filename, Scene* scene, RenderOptions&
                 std::sscanf(line, " normalTexture %99s", nTex);
                 std::sscanf(line, " emissionTexture %99s", eTex);
             }
-            // Texture maps are outside this build: a material naming one keeps
-            // texture id -1, which is what the reference stores when a texture
-            // fails to load (Scene.cpp:57-65).
+            // Texture maps (Loader.cpp:133-147): a file that cannot be read
+            // leaves the id at -1, as Scene::AddTexture does (Scene.cpp:45-66).
+            if (std::strcmp(albedoTex, "None") != 0) material.albedoTexID = (float)scene->AddTexture(path + albedoTex);
+            if (std::strcmp(mrTex, "None") != 0) material.metallicRoughnessTexID = (float)scene->AddTexture(path + mrTex);
+            if (std::strcmp(nTex, "None") != 0) material.normalmapTexID = (float)scene->AddTexture(path + nTex);
+            if (std::strcmp(eTex, "None") != 0) material.emissionmapTexID = (float)scene->AddTexture(path + eTex);
             if (!findMat(name)) {
                 int id = scene->AddMaterial(material);
                 materialMap.emplace_back(name, MaterialData{material, id});
@@ -1046,6 +1098,10 @@ bool LoadSceneFromFile(const std::string& filename, Scene* scene, RenderOptions&
         return false;
     }
     scene->CreateAccelerationStructures();
+    if (!scene->PackTextures()) {
+        if (err) *err = scene->lastError;
+        return false;
+    }
     return true;
 }
 

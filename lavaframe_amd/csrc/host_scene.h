@@ -290,6 +290,11 @@ public:
     // De-indexed triangles (3 vertices each) from arrays; normals/uvs optional.
     int AddMesh(const float* positions, const float* normals, const float* uvs,
                 int64_t num_tris, const std::string& name);
+    // Scene::AddTexture (Scene.cpp:45-66): a file (PNG / binary PPM) or RGBA8
+    // pixels; the same name twice gives the first id; -1 if the file cannot be read.
+    int AddTexture(const std::string& filename);
+    int AddTexture(const uint8_t* rgba, int width, int height, const std::string& name);
+    bool PackTextures();
     int AddMaterial(const Material& material);
     int AddMeshInstance(const MeshInstance& meshInstance);
     int AddLight(const Light& light);
@@ -312,6 +317,14 @@ public:
     std::vector<int32_t> instanceMeshes, instanceMaterials;
     bool flatOnly = false;                   // Flatten(): no BVH arrays
     std::vector<Material> materials;
+    struct Texture {
+        std::string name;
+        int width = 0, height = 0;
+        std::vector<uint8_t> rgba;
+    };
+    std::vector<Texture> textures;
+    std::vector<uint8_t> textureMapsArray;   // RGBA8 layers, first row first
+    int texWidth = 0, texHeight = 0;
     std::vector<MeshInstance> meshInstances;
     bool instancesModified = false;
     std::vector<Light> lights;

@@ -281,12 +281,6 @@ LFH_DEV void combine_nee(PathBufs& pb, int p, V3& rad) {
     }
 }
 
-LFH_DEV void onb(V3 N, V3& T, V3& B) {
-    V3 up = fabsf(N.z) < 0.999f ? v3(0.f, 0.f, 1.f) : v3(1.f, 0.f, 0.f);
-    T = normalize(cross(up, N));
-    B = cross(N, T);
-}
-
 template <bool MULTI>
 __global__ __launch_bounds__(256) void k_shade(ShadeScene sc1, const JobDev* __restrict__ jobs, int depth,
                                                const float4* __restrict__ rayq,
@@ -317,6 +311,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeScene sc1, const JobDev* __r
         float4 misc = pb.misc[p];
         float prevPdf = misc.x;
         int lastMat = __float_as_int(misc.y);
+        float2 texc = make_float2(misc.z, misc.w);   // last surface's texture coordinate
         const float t = h0.x;
         const int light = __float_as_int(h1.w);
         const int slot = __float_as_int(h0.w);
@@ -336,9 +331,10 @@ __global__ __launch_bounds__(256) void k_shade(ShadeScene sc1, const JobDev* __r
             }
         } else if (slot < 0 && light >= 0) {
             // analytic light is the closest hit (pathtrace.glsl:246-260): the
-            // material fetched is the last surface's (state is not refreshed)
+            // material fetched is the last surface's (state is not refreshed),
+            // its emission map read at that surface's texture coordinate
             Mat m = load_material(sc.materials, lastMat);
-            rad = rad + m.emission * thr;
+            rad = rad + textured_emission(sc, lastMat, make_float2(misc.z, misc.w), m.emission) * thr;
             V3 em = ld3(sc.lights[(size_t)light * kLightF4 + 1]);
             V3 Le = (depth == 0) ? em : powerHeuristic(prevPdf, h1.z) * em;
             rad = rad + Le * thr;
@@ -367,10 +363,15 @@ __global__ __launch_bounds__(256) void k_shade(ShadeScene sc1, const JobDev* __r
             float4 m0 = np[0], m1 = np[1], m2 = np[2];
             nrm = normalize(v3(m0.x * nrm.x + m0.y * nrm.y + m0.z * nrm.z, m1.x * nrm.x + m1.y * nrm.y + m1.z * nrm.z,
                                m2.x * nrm.x + m2.y * nrm.y + m2.z * nrm.z));
+            Mat m = load_material(sc.materials, matId);
+            if (sc.tex) {
+                const float2 t1 = sc.uvs[vi.x], t2 = sc.uvs[vi.y], t3 = sc.uvs[vi.z];
+                texc = make_float2(t1.x * w + t2.x * u + t3.x * v, t1.y * w + t2.y * u + t3.y * v);
+                apply_textures(sc, matId, texc, m, nrm);
+            }
             Frame fr;
             fr.N = dot(nrm, rd) <= 0.0f ? nrm : nrm * -1.0f;
             onb(nrm, fr.T, fr.B);
-            Mat m = load_material(sc.materials, matId);
             fr.eta = dot(rd, nrm) < 0.0f ? (1.0f / m.ior) : m.ior;
             V3 absorb = ld3(pb.absorb[p]);
             if (dot(nrm, fr.N) > 0.0f) absorb = v3(0.f, 0.f, 0.f);
@@ -492,7 +493,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeScene sc1, const JobDev* __r
             prevPdf = misPdf;
         }
         pb.rad[p] = f4(rad, 0.f);
-        pb.misc[p] = make_float4(prevPdf, __int_as_float(lastMat), 0.f, 0.f);
+        pb.misc[p] = make_float4(prevPdf, __int_as_float(lastMat), texc.x, texc.y);
     }
     uint32_t s = wave_append(next_count, cont);
     if (cont) {

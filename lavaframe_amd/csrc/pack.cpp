@@ -275,6 +275,30 @@ struct Builder {
         // materials
         ps->num_materials = d.num_materials;
         ps->materials.assign(d.materials, d.materials + (size_t)d.num_materials * 28);
+        // texture maps: every material's ids must name a layer (or be -1)
+        const int ntex = d.textures ? d.num_textures : 0;
+        if (ntex > 0 && (d.tex_width <= 0 || d.tex_height <= 0)) { err = "bad texture size"; return false; }
+        for (int m = 0; m < d.num_materials; ++m)
+            for (int k = 24; k < 28; ++k) {
+                const float id = d.materials[(size_t)m * 28 + k];
+                if (!(id == -1.0f || (id >= 0.0f && id < (float)ntex && id == std::floor(id)))) {
+                    err = "material " + std::to_string(m) + " names texture " + std::to_string(id) + " of " +
+                          std::to_string(ntex);
+                    return false;
+                }
+            }
+        ps->tex_layers = ntex;
+        ps->tex_w = ntex ? d.tex_width : 0;
+        ps->tex_h = ntex ? d.tex_height : 0;
+        ps->tex.assign(d.textures, d.textures + (size_t)ntex * ps->tex_w * ps->tex_h * 4);
+        ps->uvs.clear();
+        if (ntex > 0) {
+            ps->uvs.resize((size_t)d.num_vertices * 2);
+            for (int64_t v = 0; v < d.num_vertices; ++v) {
+                ps->uvs[v * 2] = d.vertices_uvx[v * 4 + 3];
+                ps->uvs[v * 2 + 1] = d.normals_uvy ? d.normals_uvy[v * 4 + 3] : 0.f;
+            }
+        }
         // lights
         ps->num_lights = d.num_lights;
         ps->lights.assign((size_t)d.num_lights * 28, 0.f);
@@ -621,6 +645,8 @@ hipError_t upload_scene(PackedScene& ps, DeviceScene* ds, hipStream_t s, std::st
     if ((e = up(ds->hdr_cols, ps.hdr_cols, s)) != hipSuccess) return e;
     if ((e = up(ds->hdr_marg, ps.hdr_marg, s)) != hipSuccess) return e;
     if ((e = up(ds->hdr_cond, ps.hdr_cond, s)) != hipSuccess) return e;
+    if ((e = up(ds->tex, ps.tex, s)) != hipSuccess) return e;
+    if ((e = up(ds->uvs, ps.uvs, s)) != hipSuccess) return e;
     if (ps.device_build) {
         std::string m = device_build(&ps, ds, s);
         if (!m.empty()) {
@@ -668,6 +694,11 @@ lfhd::ShadeScene shade_view(const PackedScene& ps, const DeviceScene& ds, const 
     s.bg[1] = o.bg_color[1];
     s.bg[2] = o.bg_color[2];
     s.estimator = o.estimator;
+    s.tex_layers = ps.tex_layers;
+    s.tex_w = ps.tex_w;
+    s.tex_h = ps.tex_h;
+    s.tex = ps.tex_layers ? ds.tex.as<const uint32_t>() : nullptr;
+    s.uvs = ps.tex_layers ? ds.uvs.as<const float2>() : nullptr;
     return s;
 }
 

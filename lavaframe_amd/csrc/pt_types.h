@@ -34,6 +34,9 @@ struct ShadeScene {
     int const_bg;             // CONSTANT_BG
     float bg[3];
     int estimator;            // 0: reference MIS weights; 1: mixture pdf after BSDF samples
+    const uint32_t* tex;      // RGBA8 texture layers (NULL: no textures)
+    const float2* uvs;        // per vertex (uv.x, uv.y) when textures exist
+    int tex_w, tex_h, tex_layers;
 };
 
 // Camera uniforms + tile / frame of one pass (renderer.glsl inputs).

@@ -65,6 +65,8 @@ uint64_t hash_desc(const lfh_scene_desc& d) {
         h = fnv(h, d.hdr_marginal, (size_t)d.hdr_height * 8);
         h = fnv(h, d.hdr_conditional, n * 8);
     }
+    if (d.textures && d.num_textures > 0)
+        h = fnv(h, d.textures, (size_t)d.num_textures * d.tex_width * d.tex_height * 4);
     if (!d.nodes && d.num_meshes > 0) {  // device BVH build: the mesh and instance tables
         h = fnv(h, d.mesh_tri_offsets, (size_t)(d.num_meshes + 1) * 8);
         h = fnv(h, d.instance_meshes, (size_t)d.num_instances * 4);
@@ -76,8 +78,8 @@ uint64_t hash_desc(const lfh_scene_desc& d) {
 int64_t packed_bytes(const PackedScene& ps) {
     int64_t b = (int64_t)(ps.nodes.size() + ps.tris.size() + ps.insts.size() + ps.xforms.size() + ps.nmats.size() +
                           ps.normals.size() + ps.materials.size() + ps.lights.size() + ps.hdr_cols.size() +
-                          ps.hdr_marg.size() + ps.hdr_cond.size()) * 4 +
-                (int64_t)ps.vidx.size() * 4;
+                          ps.hdr_marg.size() + ps.hdr_cond.size() + ps.uvs.size()) * 4 +
+                (int64_t)ps.vidx.size() * 4 + (int64_t)ps.tex.size();
     if (ps.device_build)  // resident after the build: triangle records, BVH4 nodes (at most one per 3 triangles + TLAS)
         b += ps.num_tris * 48 + (ps.num_tris / 3 + 2 * (int64_t)ps.num_instances + 2) * 128;
     return b;

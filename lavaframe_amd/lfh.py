@@ -57,6 +57,7 @@ class SceneDesc(C.Structure):
         ("hdr_cols", f32p), ("hdr_marginal", f32p), ("hdr_conditional", f32p),
         ("mesh_tri_offsets", i64p), ("num_meshes", C.c_int32),
         ("instance_meshes", i32p), ("instance_materials", i32p),
+        ("textures", u8p), ("tex_width", C.c_int32), ("tex_height", C.c_int32), ("num_textures", C.c_int32),
     ]
 
 
@@ -129,6 +130,8 @@ _SIGS = {
     "lfh_scene_rebuild_instances": (C.c_int, [_P]),
     "lfh_scene_get_desc": (C.c_int, [_P, C.POINTER(SceneDesc)]),
     "lfh_scene_flatten": (C.c_int, [_P]),
+    "lfh_scene_add_texture": (C.c_int, [_P, u8p, C.c_int32, C.c_int32, C.c_char_p, i32p]),
+    "lfh_scene_add_texture_file": (C.c_int, [_P, C.c_char_p, i32p]),
     "lfh_scene_counts": (C.c_int, [_P, i32p, i32p]),
     "lfh_create": (C.c_int, [C.POINTER(SceneDesc), C.POINTER(Options), C.POINTER(_P)]),
     "lfh_destroy": (None, [_P]),
@@ -382,6 +385,25 @@ class Scene:
         self._chk(self.lib.lfh_scene_build(self.h))
         return self
 
+    def add_texture(self, image, name: str = "") -> int:
+        """Scene::AddTexture with decoded pixels: (H, W, 4) or (H, W, 3) uint8,
+        rows top to bottom (alpha 255 when absent)."""
+        a = np.asarray(image, np.uint8)
+        if a.ndim != 3 or a.shape[2] not in (3, 4):
+            raise ValueError("texture must be (H, W, 3|4) uint8")
+        if a.shape[2] == 3:
+            a = np.concatenate([a, np.full(a.shape[:2] + (1,), 255, np.uint8)], 2)
+        a = np.ascontiguousarray(a)
+        i = C.c_int32()
+        self._chk(self.lib.lfh_scene_add_texture(self.h, a.ctypes.data_as(u8p), a.shape[1], a.shape[0],
+                                                 name.encode(), C.byref(i)))
+        return i.value
+
+    def add_texture_file(self, path) -> int:
+        i = C.c_int32()
+        self._chk(self.lib.lfh_scene_add_texture_file(self.h, str(path).encode(), C.byref(i)))
+        return i.value
+
     def set_instance_transform(self, inst: int, transform16):
         t = np.ascontiguousarray(transform16, np.float32).reshape(16)
         self._chk(self.lib.lfh_scene_set_instance_transform(self.h, int(inst), fptr(t)))
@@ -421,6 +443,9 @@ class Scene:
             "instance_meshes": a(d.instance_meshes, d.num_instances, np.int32),
             "instance_materials": a(d.instance_materials, d.num_instances, np.int32),
         }
+        if d.num_textures:
+            out["textures"] = a(d.textures, d.num_textures * d.tex_height * d.tex_width * 4, np.uint8).reshape(
+                d.num_textures, d.tex_height, d.tex_width, 4)
         if d.hdr_width:
             n = d.hdr_width * d.hdr_height
             out["hdr_cols"] = a(d.hdr_cols, n * 3, np.float32).reshape(d.hdr_height, d.hdr_width, 3)
@@ -451,6 +476,11 @@ def desc_from_arrays(arr: dict) -> tuple[SceneDesc, list]:
     d.num_meshes = max(n_off - 1, 0)
     d.instance_meshes, _ = p("instance_meshes", np.int32, i32p)
     d.instance_materials, _ = p("instance_materials", np.int32, i32p)
+    if arr.get("textures") is not None and np.asarray(arr["textures"]).size:
+        t = np.ascontiguousarray(arr["textures"], np.uint8)
+        keep.append(t)
+        d.textures = t.ctypes.data_as(u8p)
+        d.num_textures, d.tex_height, d.tex_width = t.shape[:3]
     if "hdr_cols" in arr and arr["hdr_cols"] is not None:
         cols = np.ascontiguousarray(arr["hdr_cols"], np.float32)
         d.hdr_height, d.hdr_width = cols.shape[:2]

@@ -342,3 +342,75 @@ def aperture_room(width=1280, height=720, max_depth=10, target_tris=200000, seed
     s.set_options(width=width, height=height, tile_width=width, tile_height=height, max_depth=max_depth,
                   enable_rr=1, rr_depth=2, use_env_map=0, use_constant_bg=0, hdr_multiplier=1.0)
     return s.build()
+
+
+def _quad_uv(u0, v0, u1, v1):
+    """Texture coordinates of _quad's six vertices for a quad spanning
+    [u0, u1] x [v0, v1] (p0 at (u0, v0), p2 at (u1, v1))."""
+    return [(u0, v0), (u1, v0), (u1, v1), (u0, v0), (u1, v1), (u0, v1)]
+
+
+def texture_maps(size=32):
+    """Four RGBA8 maps of one size: checker albedo, metallic (R) / roughness
+    (G) ramps, a tangent-space normal map of sinusoidal bumps, an emission
+    ring."""
+    yy, xx = np.mgrid[0:size, 0:size]
+    chk = ((xx // 4 + yy // 4) % 2)[..., None]
+    albedo = np.where(chk, np.array([230, 200, 120, 255]), np.array([60, 90, 200, 255])).astype(np.uint8)
+    mr = np.stack([xx * 255 // (size - 1), 40 + yy * 215 // (size - 1), np.zeros_like(xx),
+                   np.full_like(xx, 255)], -1).astype(np.uint8)
+    n = np.stack([0.45 * np.sin(xx * 2 * math.pi / 8), 0.45 * np.cos(yy * 2 * math.pi / 8), np.ones_like(xx, float)], -1)
+    n /= np.linalg.norm(n, axis=-1, keepdims=True)
+    nm = np.concatenate([np.round((n * 0.5 + 0.5) * 255), np.full(n.shape[:2] + (1,), 255)], -1).astype(np.uint8)
+    r = np.hypot(xx - size / 2 + 0.5, yy - size / 2 + 0.5)
+    ring = ((r > size * 0.2) & (r < size * 0.35))[..., None]
+    em = np.where(ring, np.array([255, 180, 60, 255]), np.array([10, 10, 30, 255])).astype(np.uint8)
+    return albedo, mr, nm, em
+
+
+def textured_room(width=64, height=64, max_depth=4, lib=None, build=True) -> lfh.Scene:
+    """Cornell-like box with texture maps: a checker albedo on the floor (its
+    texture coordinates run over [0, 3], so the maps repeat) and on the back
+    wall (coordinates from -0.5, negative ones wrap), a normal map on the
+    floor, metallic-roughness maps on a box, an emission map on a panel."""
+    s = lfh.Scene(lib)
+    s.add_material(lfh.MATERIAL_DEFAULT)
+    albedo, mr, nm, em = texture_maps()
+    ta, tm, tn, te = (s.add_texture(a, n) for a, n in ((albedo, "albedo"), (mr, "mr"), (nm, "normal"),
+                                                         (em, "emission")))
+    def mat(texids, **kw):
+        m = lfh.material(**kw)
+        m[24:28] = texids
+        return s.add_material(m)
+    floor_m = mat((ta, -1, tn, -1), albedo=(0.9, 0.9, 0.9), roughness=0.4)
+    back_m = mat((ta, -1, -1, -1), albedo=(0.8, 0.8, 0.8))
+    red = s.add_material(lfh.material((0.65, 0.05, 0.05)))
+    green = s.add_material(lfh.material((0.12, 0.45, 0.15)))
+    white = s.add_material(lfh.material((0.73, 0.73, 0.73)))
+    box_m = mat((-1, tm, -1, -1), albedo=(0.8, 0.6, 0.4))
+    panel_m = mat((-1, -1, -1, te), albedo=(0.5, 0.5, 0.5))
+    P = lambda x, y, z: np.array([x, y, z], np.float32)  # noqa: E731
+    floor = s.add_mesh(np.array(_quad(P(0, 0, 0), P(0, 0, 1), P(1, 0, 1), P(1, 0, 0)), np.float32),
+                       uvs=np.array(_quad_uv(0, 0, 3, 3), np.float32))
+    back = s.add_mesh(np.array(_quad(P(0, 0, 1), P(0, 1, 1), P(1, 1, 1), P(1, 0, 1)), np.float32),
+                      uvs=np.array(_quad_uv(-0.5, -0.5, 1.5, 1.5), np.float32))
+    ceil = s.add_mesh(np.array(_quad(P(0, 1, 0), P(1, 1, 0), P(1, 1, 1), P(0, 1, 1)), np.float32))
+    left = s.add_mesh(np.array(_quad(P(0, 0, 0), P(0, 1, 0), P(0, 1, 1), P(0, 0, 1)), np.float32))
+    right = s.add_mesh(np.array(_quad(P(1, 0, 0), P(1, 0, 1), P(1, 1, 1), P(1, 1, 0)), np.float32))
+    cube_uv = np.array([_quad_uv(0, 0, 1, 1)] * 6, np.float32).reshape(-1, 2)
+    cube = s.add_mesh(unit_cube_tris(), uvs=cube_uv)
+    panel = s.add_mesh(np.array(_quad(P(0.05, 0.45, 0.5), P(0.05, 0.85, 0.5), P(0.05, 0.85, 0.9),
+                                      P(0.05, 0.45, 0.9)), np.float32), uvs=np.array(_quad_uv(0, 0, 1, 1), np.float32))
+    eye = affine()
+    s.add_instance(floor, eye, floor_m)
+    s.add_instance(back, eye, back_m)
+    s.add_instance(ceil, eye, white)
+    s.add_instance(left, eye, red)
+    s.add_instance(right, eye, green)
+    s.add_instance(cube, affine(_rot_y(-17), (0.3, 0.3, 0.3), (0.36, 0.15, 0.35)), box_m)
+    s.add_instance(panel, eye, panel_m)
+    s.add_light(lfh.quad_light((0.38, 0.999, 0.40), (0.62, 0.999, 0.40), (0.38, 0.999, 0.60), (17, 12, 4)))
+    s.set_camera((0.5, 0.5, -1.35), (0.5, 0.5, 0.5), 40.0)
+    s.set_options(width=width, height=height, tile_width=width, tile_height=height, max_depth=max_depth,
+                  enable_rr=1, rr_depth=2, use_env_map=0, use_constant_bg=0, hdr_multiplier=1.0)
+    return s.build() if build else s.flatten()

@@ -51,6 +51,8 @@ def lib():
                                        f32p, C.c_int32, f32p, C.c_int32, f32p, C.c_int32, C.c_int32, C.c_int32,
                                        f32p, f32p, f32p]
         L.orc_scene_destroy.argtypes = [P]
+        L.orc_scene_set_textures.argtypes = [P, u8p, C.c_int32, C.c_int32, C.c_int32]
+        L.orc_texture_sample.argtypes = [P, f64p, i32p, C.c_int64, f64p]
         L.orc_trace_closest.restype = C.c_int64
         L.orc_trace_closest.argtypes = [P, f32p, C.c_int64, P, C.c_int, i32p, C.c_int]
         L.orc_trace_any.argtypes = [P, f32p, C.c_int64, u8p, C.c_int]
@@ -111,6 +113,20 @@ class OracleScene:
                                     fp("hdr_conditional"))
         self.num_lights = lt.shape[0]
         self.has_env = hw > 0
+        tex = arr.get("textures")
+        if tex is not None and np.asarray(tex).size:
+            t = np.ascontiguousarray(tex, np.uint8)   # (layers, H, W, 4)
+            self._tex = t
+            L.orc_scene_set_textures(self.h, t.ctypes.data_as(u8p), t.shape[2], t.shape[1], t.shape[0])
+
+    def texture_sample(self, st, layer):
+        """FP64 texture() reads: st (n, 2) texture coordinates, layer (n,)."""
+        a = np.ascontiguousarray(st, np.float64).reshape(-1, 2)
+        l = np.ascontiguousarray(np.broadcast_to(layer, (a.shape[0],)), np.int32)
+        out = np.zeros((a.shape[0], 4))
+        self.L.orc_texture_sample(self.h, a.ctypes.data_as(f64p), l.ctypes.data_as(i32p), a.shape[0],
+                                  out.ctypes.data_as(f64p))
+        return out
 
     def __del__(self):
         try:

@@ -99,6 +99,8 @@ struct orc_scene {
     int32_t nmat = 0, ninst = 0, nlights = 0;
     int32_t hw = 0, hh = 0;
     std::vector<float> hcols, hmarg, hcond;
+    std::vector<uint8_t> tex;   // RGBA8 texture layers, first row first
+    int32_t tw = 0, th = 0, tl = 0;
 };
 
 namespace {
@@ -258,6 +260,7 @@ template <class T> struct State {
     V3<T> fhp, normal, ffnormal, tangent, bitangent;
     bool isEmitter;
     V3<T> bary;
+    T texCoord[2];
     int triID[3];
     int matID;
     int inst;
@@ -864,11 +867,15 @@ template <class T> Material<T> material_of(const orc_scene* s, int id) {
     return m;
 }
 
-// GetNormalsAndTexCoord (pathtrace.glsl:16-37) without the unused tex coords
+// GetNormalsAndTexCoord (pathtrace.glsl:16-37)
 template <class T> void GetNormals(const orc_scene* s, State<T>& st, const Ray<T>& r, const M4<T>& xf) {
     const float* n1 = &s->nrms[(size_t)st.triID[0] * 4];
     const float* n2 = &s->nrms[(size_t)st.triID[1] * 4];
     const float* n3 = &s->nrms[(size_t)st.triID[2] * 4];
+    const float u1 = s->verts[(size_t)st.triID[0] * 4 + 3], u2 = s->verts[(size_t)st.triID[1] * 4 + 3],
+                u3 = s->verts[(size_t)st.triID[2] * 4 + 3];
+    st.texCoord[0] = (T)u1 * st.bary.x + (T)u2 * st.bary.y + (T)u3 * st.bary.z;
+    st.texCoord[1] = (T)n1[3] * st.bary.x + (T)n2[3] * st.bary.y + (T)n3[3] * st.bary.z;
     V3<T> a = mk<T>(n1[0], n1[1], n1[2]), b = mk<T>(n2[0], n2[1], n2[2]), c = mk<T>(n3[0], n3[1], n3[2]);
     V3<T> normal = normalize(a * st.bary.x + b * st.bary.y + c * st.bary.z);
     normal = normalize(normal_xform(xf, normal));
@@ -877,9 +884,59 @@ template <class T> void GetNormals(const orc_scene* s, State<T>& st, const Ray<T
     Onb(st.normal, st.tangent, st.bitangent);
 }
 
-// GetMaterialsAndTextures (pathtrace.glsl:40-123) without texture maps
+// texture() on the RGBA8 array texture (Renderer.cpp:151-160): GL_LINEAR,
+// GL_REPEAT (the default wrap), layer clamped to the array (OpenGL 4.6,
+// 8.14.2); unsigned normalized channels c / 255.
+template <class T> void tex_rgba8(const orc_scene* s, T sx, T ty, int layer, T out[4]) {
+    const int W = s->tw, H = s->th;
+    layer = layer < 0 ? 0 : (layer >= s->tl ? s->tl - 1 : layer);
+    T u = sx * (T)W - T(0.5), v = ty * (T)H - T(0.5);
+    T fu = std::floor(u), fv = std::floor(v);
+    T a = u - fu, b = v - fv;
+    fu -= (T)W * std::floor(fu / (T)W);
+    fv -= (T)H * std::floor(fv / (T)H);
+    int i0 = wrapi((int)fu, W), i1 = wrapi((int)fu + 1, W);
+    int j0 = wrapi((int)fv, H), j1 = wrapi((int)fv + 1, H);
+    const uint8_t* L = &s->tex[(size_t)layer * W * H * 4];
+    auto px = [&](int i, int j, int c) { return (T)L[((size_t)j * W + i) * 4 + c] / T(255); };
+    for (int c = 0; c < 4; ++c) {
+        T top = px(i0, j0, c) * (T(1) - a) + px(i1, j0, c) * a;
+        T bot = px(i0, j1, c) * (T(1) - a) + px(i1, j1, c) * a;
+        out[c] = top * (T(1) - b) + bot * b;
+    }
+}
+
+// GetMaterialsAndTextures (pathtrace.glsl:40-123)
 template <class T> void GetMaterial(const orc_scene* s, State<T>& st, const Ray<T>& r) {
     st.mat = material_of<T>(s, st.matID);
+    if (s->tl > 0) {
+        const float* ids = &s->mats[(size_t)st.matID * 28 + 24];
+        const T sx = st.texCoord[0], ty = T(1) - st.texCoord[1];
+        T c[4];
+        auto pw = [](T x) { return std::pow(x, T(2.2)); };
+        if ((int)ids[0] >= 0) {
+            tex_rgba8<T>(s, sx, ty, (int)ids[0], c);
+            st.mat.albedo = st.mat.albedo * mk(pw(c[0]), pw(c[1]), pw(c[2]));
+        }
+        if ((int)ids[1] >= 0) {
+            tex_rgba8<T>(s, sx, ty, (int)ids[1], c);
+            st.mat.metallic = c[0];
+            st.mat.roughness = std::fmax(c[1] * c[1], T(0.001));
+        }
+        if ((int)ids[2] >= 0) {
+            tex_rgba8<T>(s, sx, ty, (int)ids[2], c);
+            V3<T> n = normalize(mk(c[0] * T(2) - T(1), c[1] * T(2) - T(1), c[2] * T(2) - T(1)));
+            V3<T> Tg, B;
+            Onb(st.normal, Tg, B);
+            st.normal = normalize(Tg * n.x + B * n.y + st.normal * n.z);
+            st.ffnormal = dot(st.normal, r.direction) <= T(0) ? st.normal : st.normal * T(-1);
+            Onb(st.normal, st.tangent, st.bitangent);
+        }
+        if (ids[3] >= 0.0f) {
+            tex_rgba8<T>(s, sx, ty, (int)ids[3], c);
+            st.mat.emission = mk(pw(c[0]), pw(c[1]), pw(c[2]));
+        }
+    }
     st.eta = dot(r.direction, st.normal) < T(0) ? (T(1) / st.mat.ior) : st.mat.ior;
 }
 
@@ -1106,6 +1163,17 @@ orc_scene* orc_scene_create(const float* nodes, int64_t num_nodes, int32_t top, 
         s->hcond.assign(hdr_conditional, hdr_conditional + (size_t)hdr_w * hdr_h * 2);
     }
     return s;
+}
+
+void orc_texture_sample(const orc_scene* s, const double* st, const int32_t* layer, int64_t n, double* out4) {
+    for (int64_t i = 0; i < n; ++i) tex_rgba8<double>(s, st[i * 2], st[i * 2 + 1], layer[i], out4 + i * 4);
+}
+
+void orc_scene_set_textures(orc_scene* s, const uint8_t* rgba8, int32_t width, int32_t height, int32_t layers) {
+    s->tw = width;
+    s->th = height;
+    s->tl = layers;
+    s->tex.assign(rgba8, rgba8 + (size_t)width * height * layers * 4);
 }
 
 void orc_scene_destroy(orc_scene* s) { delete s; }

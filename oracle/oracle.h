@@ -47,6 +47,11 @@ orc_scene* orc_scene_create(const float* nodes, int64_t num_nodes, int32_t top_l
                             const float* lights, int32_t num_lights,
                             int32_t hdr_w, int32_t hdr_h, const float* hdr_cols,
                             const float* hdr_marginal, const float* hdr_conditional);
+/* Texture maps: RGBA8 layers of width x height, first row first
+ * (textureMapsArray, Scene.cpp:223-230). */
+void orc_scene_set_textures(orc_scene* s, const uint8_t* rgba8, int32_t width, int32_t height, int32_t layers);
+/* texture() reads (FP64) of layer[i] at texture coordinates st[2i], st[2i+1]. */
+void orc_texture_sample(const orc_scene* s, const double* st, const int32_t* layer, int64_t n, double* out4);
 void orc_scene_destroy(orc_scene* s);
 
 /* ClosestHit over the BVH only (closest_hit.glsl:69-204, no analytic lights),
