@@ -282,6 +282,14 @@ int  lfh_get_accum(lfh_ctx* ctx, float* sum, float* comp, int32_t w, int32_t h);
 /* Render `passes` full passes back to back (each = Update(no reset) + Render
  * over all tiles), graph-captured; for benchmarks and serving. */
 int  lfh_render_passes(lfh_ctx* ctx, int32_t passes);
+/* The preview engine's frame (preview_flareon.glsl; TiledRenderer::Render
+ * draws it while the camera moves or instances change, TiledRenderer.cpp:
+ * 327-333): one sample per pixel of a (width * scale) x (height * scale)
+ * frame with the current camera, RNG frame 1, path depth 2, no
+ * accumulation; use_dof: thin-lens rays (USE_DOF, off by default in the
+ * reference).  dst (may be NULL): w * h * 3 floats, rows bottom to top;
+ * *w, *h receive the size.  The accumulation is left untouched. */
+int  lfh_render_preview(lfh_ctx* ctx, float scale, int32_t use_dof, float* dst, int32_t* w, int32_t* h);
 
 /* ---- direct traversal (microbenchmark + parity) -------------------------- */
 /* Host arrays: copies in, traces, copies out, synchronises. */

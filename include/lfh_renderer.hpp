@@ -44,6 +44,19 @@ public:
     // Build the BVHs on the device at Init (from the scene's meshes and
     // instances) instead of uploading the scene's own BVH arrays.
     void SetDeviceBvh(bool on) { device_bvh_ = on; }
+    // GlobalState.previewScale and useDofInPreview (GlobalState.h, Main.cpp:525-527).
+    void SetPreview(float scale, bool use_dof) {
+        preview_scale_ = scale;
+        preview_dof_ = use_dof;
+    }
+    // The last preview frame drawn by Render() while the camera moved:
+    // linear RGB, rows bottom to top (what Present shows after the post pass).
+    void GetPreviewBufferHDR(float** data, int& w, int& h) {
+        *data = new float[preview_.size()];
+        std::memcpy(*data, preview_.data(), preview_.size() * sizeof(float));
+        w = preview_w_;
+        h = preview_h_;
+    }
 
     virtual void Init() {
         if (ctx_) return;
@@ -82,11 +95,13 @@ public:
         }
         lfh_camera c = camera();
         check(lfh_update(ctx_, &c, reset ? 1 : 0), "Update");
+        moving_ = reset;
     }
 
     virtual void Render() {
         if (!ctx_) return;
-        check(lfh_render(ctx_), "Render");
+        if (moving_) render_preview();
+        else check(lfh_render(ctx_), "Render");
     }
 
     virtual void Present() const {}
@@ -238,11 +253,26 @@ protected:
     template <class V> static const typename V::value_type* hdr_ptr(const V& v) { return v.data(); }
     template <class H> static const float* hdr_cols(const H& h) { return hdr_ptr(h.cols); }
 
+    // Render() while the camera moves: the preview engine's frame
+    // (TiledRenderer.cpp:327-333) instead of a path-traced tile.
+    void render_preview() {
+        int32_t w = (int32_t)((float)width_ * preview_scale_), h = (int32_t)((float)height_ * preview_scale_);
+        preview_.resize((size_t)(w > 0 ? w : 0) * (h > 0 ? h : 0) * 3);
+        check(lfh_render_preview(ctx_, preview_scale_, preview_dof_ ? 1 : 0, preview_.data(), &w, &h), "Render");
+        preview_w_ = w;
+        preview_h_ = h;
+    }
+
     SceneT* scene_;
     std::string shadersDirectory_;
     lfh_ctx* ctx_ = nullptr;
     int width_ = 0, height_ = 0;
     bool device_bvh_ = false;
+    bool moving_ = false;
+    float preview_scale_ = 1.0f;
+    bool preview_dof_ = false;
+    std::vector<float> preview_;
+    int preview_w_ = 0, preview_h_ = 0;
     mutable std::vector<int64_t> mesh_offsets_;
     mutable std::vector<int32_t> inst_mesh_, inst_mat_;
 };

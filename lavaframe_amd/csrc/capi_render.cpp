@@ -94,7 +94,7 @@ constexpr int kMaxBatch = 64;
 // (frames frameCounter .. frameCounter + batch - 1), accumulated per pixel in
 // frame order.
 int render_tile(lfh_ctx* c, int batch) {
-    lfhd::CamParams cp;
+    lfhd::CamParams cp{};
     std::memcpy(cp.pos, c->cam.position, 12);
     std::memcpy(cp.right, c->cam.right, 12);
     std::memcpy(cp.up, c->cam.up, 12);
@@ -165,9 +165,93 @@ void advance_counters(lfh_ctx* c, bool reset) {
     }
 }
 
+// The preview engine's frame (preview_flareon.glsl, drawn by
+// TiledRenderer::Render while the camera moves, TiledRenderer.cpp:327-333):
+// one sample per pixel of a (width x scale) x (height x scale) frame, RNG
+// frame 1, path depth 2 (the maxDepth uniform while moving,
+// TiledRenderer.cpp:533), no accumulation.
+int render_preview(lfh_ctx* c, float scale, int dof, int* pw, int* ph) {
+    const int W = (int)((float)c->opt.width * scale), H = (int)((float)c->opt.height * scale);
+    if (W <= 0 || H <= 0 || (int64_t)W * H > (int64_t)1 << 26)
+        return rfail(c, LFH_E_INVALID, "lfh_render_preview: preview size out of range");
+    const int64_t n = (int64_t)W * H;
+    if (c->pass.cap < n) {
+        hipError_t e = hipStreamSynchronize(c->stream);
+        if (e == hipSuccess) e = c->pass.alloc((int)n);
+        if (e != hipSuccess) return rhip(c, e, "lfh_render_preview: buffers");
+    }
+    // the frame, then a zero compensation buffer for the readback kernel
+    if (c->preview.bytes < (size_t)n * 32) RCHK(c, c->preview.alloc((size_t)n * 32), "lfh_render_preview: alloc");
+    RCHK(c, pt_clear4(c->preview.as<float4>(), 2 * n, c->stream), "lfh_render_preview: clear");
+    lfhd::CamParams cp{};
+    std::memcpy(cp.pos, c->cam.position, 12);
+    std::memcpy(cp.right, c->cam.right, 12);
+    std::memcpy(cp.up, c->cam.up, 12);
+    std::memcpy(cp.fwd, c->cam.forward, 12);
+    cp.fov = c->cam.fov;
+    cp.focal_dist = c->cam.focal_dist;
+    cp.aperture = c->cam.aperture;
+    cp.width = c->opt.width;     // screenResolution (jitter and aspect)
+    cp.height = c->opt.height;
+    cp.tile_w = W;
+    cp.tile_h = H;
+    cp.frame = 1;
+    cp.batch = 1;
+    cp.preview = 1;
+    cp.dof = dof;
+    PassBufs& pb = c->pass;
+    lfh_options o = c->opt;
+    o.max_depth = 2;
+    lfhd::ShadeScene sc = shade_view(c->ps, c->ds, o);
+    uint32_t* cnt = pb.counters.as<uint32_t>();
+    unsigned long long* work = pb.work.as<unsigned long long>();
+    RCHK(c, pt_reset(cnt, work, c->stream), "lfh_render_preview: reset");
+    RCHK(c, pt_generate(cp, lfh::view(pb), pb.rq0.as<float4>(), cnt + 0, c->stream), "lfh_render_preview: generate");
+    GuideHook gh{};
+    for (int depth = 0; depth < o.max_depth; ++depth) {
+        const int cur = depth & 1;
+        float4* q[2] = {pb.rq0.as<float4>(), pb.rq1.as<float4>()};
+        RCHK(c,
+             pt_bounce(sc, depth, pb.cap, q[cur], cnt + cur, q[cur ^ 1], cnt + (cur ^ 1), pb.hq.as<float4>(),
+                       pb.sq.as<float4>(), cnt + 2, lfh::view(pb), c->spill.as<uint32_t>(), work, c->errflag.as<int>(),
+                       c->cull, c->pt_grid, gh, nullptr, c->stream),
+             "lfh_render_preview: bounce");
+    }
+    lfhd::CamParams ca = cp;   // accumulate over the preview frame itself
+    ca.width = W;
+    ca.height = H;
+    RCHK(c, pt_accumulate(lfh::view(pb), ca, c->preview.as<float4>(), c->preview.as<float4>(), 0, c->stream),
+         "lfh_render_preview: output");
+    *pw = W;
+    *ph = H;
+    return LFH_OK;
+}
+
 }  // namespace lfh
 
 extern "C" {
+
+int lfh_render_preview(lfh_ctx* c, float scale, int32_t use_dof, float* dst, int32_t* w, int32_t* h) {
+    if (!c || !w || !h) return rfail(c, LFH_E_INVALID, "lfh_render_preview: NULL argument");
+    if (!c->has_cam) return rfail(c, LFH_E_STATE, "lfh_render_preview: lfh_update has not been called");
+    if (!(scale > 0.0f) || scale > 4.0f) return rfail(c, LFH_E_INVALID, "lfh_render_preview: scale must be in (0, 4]");
+    int W = 0, H = 0;
+    int r = lfh::render_preview(c, scale, use_dof ? 1 : 0, &W, &H);
+    if (r != LFH_OK) return r;
+    *w = W;
+    *h = H;
+    if (dst) {
+        const int64_t n = (int64_t)W * H;
+        const size_t bytes = (size_t)n * 3 * sizeof(float);
+        if (c->out.bytes < bytes) RCHK(c, c->out.alloc(bytes), "lfh_render_preview: alloc");
+        RCHK(c, pt_output(c->preview.as<float4>(), c->preview.as<float4>() + n, n, 1.0f, c->out.as<float>(),
+                          c->stream),
+             "lfh_render_preview: output");
+        RCHK(c, hipMemcpyAsync(dst, c->out.p, bytes, hipMemcpyDeviceToHost, c->stream), "lfh_render_preview: copy");
+    }
+    RCHK(c, hipStreamSynchronize(c->stream), "lfh_render_preview: sync");
+    return LFH_OK;
+}
 
 int lfh_update(lfh_ctx* c, const lfh_camera* cam, int reset) {
     if (!c || !cam) return rfail(c, LFH_E_INVALID, "lfh_update: NULL argument");

@@ -175,6 +175,7 @@ struct lfh_ctx {
     int device = 0;
     // traversal scratch
     lfh::DevBuf spill, work, errflag;
+    lfh::DevBuf preview;          // float4 per preview pixel (lfh_render_preview)
     int grid_blocks = 0;
     float cull = 1.0f + 1.0f / 1024.0f;
     // host staging for the host-array entry points

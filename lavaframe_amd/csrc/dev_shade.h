@@ -538,4 +538,42 @@ LFH_DEV void camera_ray(const CamParams& c, int lx, int ly, V3& org, V3& dir, ui
     org = pos + ap;
 }
 
+// Camera ray of the preview engine (preview_flareon.glsl:20-60) for pixel
+// (x, y) of a tile_w x tile_h preview of a width x height screen: the seed is
+// the preview pixel with frame 1, the jitter is divided by the screen size
+// (not half of it), the lens samples are drawn but used only with USE_DOF.
+LFH_DEV void camera_ray_preview(const CamParams& c, int x, int y, V3& org, V3& dir, uint4& seed) {
+    float W = (float)c.width, H = (float)c.height;
+    float tcx = ((float)x + 0.5f) / (float)c.tile_w;
+    float tcy = ((float)y + 0.5f) / (float)c.tile_h;
+    uint32_t ux = (uint32_t)((float)x + 0.5f), uy = (uint32_t)((float)y + 0.5f);
+    seed = make_uint4(ux, uy, 1u, ux + uy);
+    float r1 = 2.0f * rnd(seed);
+    float r2 = 2.0f * rnd(seed);
+    float jx = r1 < 1.0f ? sqrtf(r1) - 1.0f : 1.0f - sqrtf(2.0f - r1);
+    float jy = r2 < 1.0f ? sqrtf(r2) - 1.0f : 1.0f - sqrtf(2.0f - r2);
+    jx = jx / W;
+    jy = jy / H;
+    float dx = (2.0f * tcx - 1.0f) + jx, dy = (2.0f * tcy - 1.0f) + jy;
+    float scale = tanf(c.fov * 0.5f);
+    dy = dy * (H / W * scale);
+    dx = dx * scale;
+    V3 right = v3(c.right[0], c.right[1], c.right[2]);
+    V3 up = v3(c.up[0], c.up[1], c.up[2]);
+    V3 fwd = v3(c.fwd[0], c.fwd[1], c.fwd[2]);
+    V3 pos = v3(c.pos[0], c.pos[1], c.pos[2]);
+    V3 rayDir = normalize(dx * right + dy * up + fwd);
+    V3 focalPoint = c.focal_dist * rayDir;
+    float cam_r1 = rnd(seed) * kTwoPi;
+    float cam_r2 = rnd(seed) * c.aperture;
+    if (c.dof) {
+        V3 ap = (cosf(cam_r1) * right + sinf(cam_r1) * up) * sqrtf(cam_r2);
+        dir = normalize(focalPoint - ap);
+        org = pos + ap;
+    } else {
+        dir = normalize(focalPoint);
+        org = pos;
+    }
+}
+
 }  // namespace lfhd

@@ -87,12 +87,16 @@ __global__ __launch_bounds__(256) void k_generate(CamParams cam1, const JobDev* 
     if (inrange && lp < P) {
         lx = lp % cam.tile_w;
         ly = lp / cam.tile_w;
-        valid = (cam.tileX * cam.tile_w + lx < cam.width) && (cam.tileY * cam.tile_h + ly < cam.height);
+        valid = cam.preview ||
+                ((cam.tileX * cam.tile_w + lx < cam.width) && (cam.tileY * cam.tile_h + ly < cam.height));
     }
     V3 o = v3(0.f, 0.f, 0.f), d = v3(0.f, 0.f, 1.f);
     uint4 seed = make_uint4(0, 0, 0, 0);
     if (valid) {
-        camera_ray(cam, lx, ly, o, d, seed);
+        if (cam.preview)
+            camera_ray_preview(cam, lx, ly, o, d, seed);
+        else
+            camera_ray(cam, lx, ly, o, d, seed);
         pb.rad[p] = make_float4(0.f, 0.f, 0.f, 0.f);
         pb.thr[p] = make_float4(1.f, 1.f, 1.f, 0.f);
         pb.absorb[p] = make_float4(0.f, 0.f, 0.f, 0.f);

@@ -46,6 +46,8 @@ struct CamParams {
     int width, height, tile_w, tile_h, tileX, tileY;
     int frame;
     int batch;   // samples per pixel in this wavefront: path p is pixel p % P of frame + p / P
+    int preview; // 1: the preview engine's camera (tile_w x tile_h = preview size, width x height = screen)
+    int dof;     // preview only: thin-lens rays (USE_DOF)
 };
 
 // One job of a multi-job (serving) pass: its scene, camera and frame, its

@@ -146,6 +146,7 @@ _SIGS = {
     "lfh_get_output_hdr": (C.c_int, [_P, f32p, C.c_int32, C.c_int32]),
     "lfh_get_accum": (C.c_int, [_P, f32p, f32p, C.c_int32, C.c_int32]),
     "lfh_render_passes": (C.c_int, [_P, C.c_int32]),
+    "lfh_render_preview": (C.c_int, [_P, C.c_float, C.c_int32, f32p, i32p, i32p]),
     "lfh_trace_closest": (C.c_int, [_P, _P, C.c_int64, _P]),
     "lfh_trace_any": (C.c_int, [_P, _P, C.c_int64, _P]),
     "lfh_trace_closest_device": (C.c_int, [_P, _P, C.c_int64, _P]),
@@ -583,6 +584,18 @@ class Context:
 
     def render_passes(self, passes: int):
         self._chk(self.lib.lfh_render_passes(self.h, passes))
+
+    def render_preview(self, scale: float = 1.0, dof: bool = False) -> np.ndarray:
+        """The preview engine's frame at (width * scale, height * scale):
+        (h, w, 3) float32, rows bottom to top."""
+        w = int(np.float32(self.options.width) * np.float32(scale))
+        h = int(np.float32(self.options.height) * np.float32(scale))
+        out = np.zeros((max(h, 1), max(w, 1), 3), np.float32)
+        ow, oh = C.c_int32(), C.c_int32()
+        self._chk(self.lib.lfh_render_preview(self.h, float(scale), 1 if dof else 0, fptr(out), C.byref(ow),
+                                              C.byref(oh)))
+        assert (ow.value, oh.value) == (w, h)
+        return out
 
     def sample_count(self) -> int:
         v = C.c_int32()

@@ -53,6 +53,7 @@ def lib():
         L.orc_scene_destroy.argtypes = [P]
         L.orc_scene_set_textures.argtypes = [P, u8p, C.c_int32, C.c_int32, C.c_int32]
         L.orc_texture_sample.argtypes = [P, f64p, i32p, C.c_int64, f64p]
+        L.orc_preview.argtypes = [P, C.POINTER(Options), C.c_int32, C.c_int32, C.c_int32, f64p, C.c_int, C.c_int]
         L.orc_trace_closest.restype = C.c_int64
         L.orc_trace_closest.argtypes = [P, f32p, C.c_int64, P, C.c_int, i32p, C.c_int]
         L.orc_trace_any.argtypes = [P, f32p, C.c_int64, u8p, C.c_int]
@@ -118,6 +119,13 @@ class OracleScene:
             t = np.ascontiguousarray(tex, np.uint8)   # (layers, H, W, 4)
             self._tex = t
             L.orc_scene_set_textures(self.h, t.ctypes.data_as(u8p), t.shape[2], t.shape[1], t.shape[0])
+
+    def preview(self, opts: "Options", pw: int, ph: int, dof=False, precision=32, nthreads=None):
+        """The preview engine's frame: (ph, pw, 3), one sample per pixel."""
+        out = np.zeros((ph, pw, 3), np.float64)
+        self.L.orc_preview(self.h, C.byref(opts), pw, ph, 1 if dof else 0, out.ctypes.data_as(f64p), precision,
+                           nthreads or threads())
+        return out
 
     def texture_sample(self, st, layer):
         """FP64 texture() reads: st (n, 2) texture coordinates, layer (n,)."""

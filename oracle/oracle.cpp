@@ -1106,6 +1106,46 @@ void camera_ray(const orc_options* o, int tileX, int tileY, int lx, int ly, int 
     ray.direction = fin;
 }
 
+// preview_flareon.glsl:20-60 for pixel (x, y) of a pw x ph preview of the
+// o->width x o->height screen
+template <class T>
+void camera_ray_preview(const orc_options* o, int pw, int ph, int x, int y, int dof, Ray<T>& ray, uint32_t* seed) {
+    float W = (float)o->width, H = (float)o->height;
+    float tcx = ((float)x + 0.5f) / (float)pw;
+    float tcy = ((float)y + 0.5f) / (float)ph;
+    const uint32_t ux = (uint32_t)((float)x + 0.5f), uy = (uint32_t)((float)y + 0.5f);
+    seed[0] = ux;
+    seed[1] = uy;
+    seed[2] = 1u;
+    seed[3] = ux + uy;
+    T r1 = T(2) * rnd<T>(seed);
+    T r2 = T(2) * rnd<T>(seed);
+    T jx = r1 < T(1) ? std::sqrt(r1) - T(1) : T(1) - std::sqrt(T(2) - r1);
+    T jy = r2 < T(1) ? std::sqrt(r2) - T(1) : T(1) - std::sqrt(T(2) - r2);
+    jx = jx / (T)W;
+    jy = jy / (T)H;
+    T dx = (T(2) * (T)tcx - T(1)) + jx, dy = (T(2) * (T)tcy - T(1)) + jy;
+    T scale = std::tan((T)o->fov * T(0.5));
+    dy = dy * ((T)H / (T)W * scale);
+    dx = dx * scale;
+    V3<T> right = mk<T>(o->cam_right[0], o->cam_right[1], o->cam_right[2]);
+    V3<T> up = mk<T>(o->cam_up[0], o->cam_up[1], o->cam_up[2]);
+    V3<T> fwd = mk<T>(o->cam_fwd[0], o->cam_fwd[1], o->cam_fwd[2]);
+    V3<T> pos = mk<T>(o->cam_pos[0], o->cam_pos[1], o->cam_pos[2]);
+    V3<T> rayDir = normalize(dx * right + dy * up + fwd);
+    V3<T> focalPoint = (T)o->focal_dist * rayDir;
+    T cam_r1 = rnd<T>(seed) * C<T>::TWO_PI;
+    T cam_r2 = rnd<T>(seed) * (T)o->aperture;
+    if (dof) {
+        V3<T> ap = (std::cos(cam_r1) * right + std::sin(cam_r1) * up) * std::sqrt(cam_r2);
+        ray.origin = pos + ap;
+        ray.direction = normalize(focalPoint - ap);
+    } else {
+        ray.origin = pos;
+        ray.direction = normalize(focalPoint);
+    }
+}
+
 template <class T> State<T> frame_state(const orc_scene* s, const float* fr) {
     State<T> st{};
     st.normal = mk<T>(fr[0], fr[1], fr[2]);
@@ -1405,6 +1445,28 @@ void orc_render(const orc_scene* s, const orc_options* o, int32_t frame0, int32_
             }
         }
     }
+}
+
+void orc_preview(const orc_scene* s, const orc_options* o, int32_t pw, int32_t ph, int32_t dof, double* out3,
+                 int precision, int threads) {
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 16) collapse(2)
+    for (int y = 0; y < ph; ++y)
+        for (int x = 0; x < pw; ++x) {
+            uint32_t seed[4];
+            double* a = out3 + ((size_t)y * pw + x) * 3;
+            if (precision == 64) {
+                Ray<double> r;
+                camera_ray_preview<double>(o, pw, ph, x, y, dof, r, seed);
+                V3<double> c = PathTrace<double>(s, o, r, seed);
+                a[0] = c.x, a[1] = c.y, a[2] = c.z;
+            } else {
+                Ray<float> r;
+                camera_ray_preview<float>(o, pw, ph, x, y, dof, r, seed);
+                V3<float> c = PathTrace<float>(s, o, r, seed);
+                a[0] = c.x, a[1] = c.y, a[2] = c.z;
+            }
+        }
 }
 
 void orc_render_pixels(const orc_scene* s, const orc_options* o, const int32_t* pixels, int64_t npix, int32_t frame0,

@@ -105,6 +105,10 @@ void orc_path_trace_rays(const orc_scene* s, const orc_options* o, const float* 
 /* renderer.glsl:27-62 camera rays of one tile (origin, direction per tile
  * pixel, row-major), for the given tile and frame. */
 void orc_camera_rays(const orc_options* o, int32_t tileX, int32_t tileY, int32_t frame, double* rays6, int precision);
+/* The preview engine (preview_flareon.glsl) over a pw x ph preview of the
+ * o->width x o->height screen, one sample per pixel, o->max_depth as given. */
+void orc_preview(const orc_scene* s, const orc_options* o, int32_t pw, int32_t ph, int32_t dof, double* out3,
+                 int precision, int threads);
 void orc_render(const orc_scene* s, const orc_options* o, int32_t frame0, int32_t passes, double* accum,
                 int precision, int threads);
 /* Same, for a subset of pixels only (pixel ids y*w + x), accumulating per
