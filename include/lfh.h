@@ -85,6 +85,8 @@ typedef struct lfh_hit {
  *   lights       15 floats / light {pos, emission, u, v, radius, area, type}
  *   hdr_*        env radiance RGB (3 floats/texel) + the two inverse-CDF tables
  *                (2 floats / entry), hdrloader.h:19-28.  hdr_width == 0: no env.
+ *                With both tables NULL they are built on the device from the
+ *                radiance (HDRLoader::buildDistributions, bit-identical).
  * All pointers are borrowed for the duration of lfh_create only.
  *
  * Device BVH build (no reference counterpart): with nodes == NULL the BVHs are

@@ -43,6 +43,7 @@ struct PackedScene {
     uint32_t root4 = lfhd::kRefSentinel;
     int stack_need = 0;           // worst-case traversal stack entries of nodes4
     std::vector<float> tris;      // 12 floats / triangle slot
+    std::vector<float> tri_box;   // 6 floats / triangle slot: its vertices' box (host)
     std::vector<float> insts;     // 16 floats / instance (inverse rows + meta)
     std::vector<float> xforms;    // 16 floats / instance (forward, column-major)
     std::vector<float> nmats;     // 12 floats / instance (normal matrix rows)
@@ -54,6 +55,7 @@ struct PackedScene {
     std::vector<float> hdr_marg;  // 2 floats / row
     std::vector<float> hdr_cond;  // 2 floats / texel
     int hdr_w = 0, hdr_h = 0;
+    bool hdr_tables_on_device = false;   // desc without HDR tables: built at upload
     std::vector<uint8_t> tex;     // RGBA8 texture layers
     int tex_w = 0, tex_h = 0, tex_layers = 0;
     std::vector<float> uvs;       // 2 floats / vertex, only with textures

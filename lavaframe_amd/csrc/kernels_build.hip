@@ -15,8 +15,8 @@
 //  4. a top-down collapse, one tree level per pass, that turns each binary
 //     node into a BVH4 node with the widening rule of the host path (replace
 //     the internal child of largest surface area by its two children until
-//     there are four) and turns subtrees of at most kLeafTris triangles into
-//     leaves, writing their triangle records in leaf order.
+//     there are four); every triangle is a leaf of its own, its record
+//     written at its vert_indices slot.
 //
 // Allocation of merged and BVH4 nodes uses prefix sums, so a build is
 // deterministic.  Closest hits do not depend on the tree: a hit is the nearest
@@ -39,14 +39,12 @@ namespace {
 #ifndef LFH_PLOC_RADIUS
 #define LFH_PLOC_RADIUS 8
 #endif
-#ifndef LFH_LEAF_TRIS
-#define LFH_LEAF_TRIS 1
-#endif
 // On the 1M-triangle soup, radius 8 with one triangle per leaf traces fastest
-// of radius 4-32 and leaves of 1-4 triangles.
+// of radius 4-32 and leaves of 1-4 triangles.  With one triangle per leaf the
+// triangle records stay in vert_indices order (record k = triangle k, as the
+// host-built layout has them), so hits' slots index the records too.
 constexpr int kBlk = 256;
 constexpr int kRadius = LFH_PLOC_RADIUS;   // PLOC search radius
-constexpr int kLeafTris = LFH_LEAF_TRIS;   // largest BLAS leaf (the node ref holds up to 7)
 
 LFH_DEV uint32_t f2ord(float f) {
     const uint32_t u = __float_as_uint(f);
@@ -70,12 +68,6 @@ LFH_DEV uint64_t spread16(uint32_t x) {
     return v;
 }
 
-// Boxes written into BVH4 nodes are one float step wider on every side: the
-// triangle test uses v0 + (v1 - v0) rounded, which may lie half an ulp outside
-// the box of the original vertices.  One step on every box keeps each box
-// inside its parent's.  (Keys and clustering use the exact boxes.)
-LFH_DEV float down(float x) { return nextafterf(x, -__builtin_inff()); }
-LFH_DEV float up(float x) { return nextafterf(x, __builtin_inff()); }
 
 LFH_DEV float half_area(float4 lo, float4 hi) {
     const float dx = hi.x - lo.x, dy = hi.y - lo.y, dz = hi.z - lo.z;
@@ -296,7 +288,6 @@ struct CollapseArgs {
     const int2* nkids;    // internal binary nodes (ids >= nprim)
     const int* ncount;    // primitives below each binary node
     int nprim;            // binary nodes < nprim are primitives
-    int leaf_max;         // subtrees of at most this many primitives become leaves
     int tlas;             // 1: primitives are instances (leaf refs to instances)
     // triangle records (BLAS)
     const float4* verts;
@@ -316,38 +307,21 @@ struct Level {
 
 LFH_DEV bool is_internal(const CollapseArgs& a, int n) { return n >= a.nprim; }
 
-// Triangle records of the subtree of binary node n, leaves left to right, at
-// record slots o, o+1, ...: (v0, prim id), (v1 - v0, vert_indices slot), (v2 - v0, 0).
-LFH_DEV void write_leaf_tris(const CollapseArgs& a, int n, int o) {
-    int st[2 * kLeafTris + 2];
-    int sp = 0, w = 0;
-    st[sp++] = n;
-    while (sp > 0) {
-        const int x = st[--sp];
-        if (is_internal(a, x)) {
-            const int2 k = a.nkids[x];
-            st[sp++] = k.y;
-            st[sp++] = k.x;
-            continue;
-        }
-        const int4 vi = a.vidx[x];
-        const float4 v0 = a.verts[vi.x], v1 = a.verts[vi.y], v2 = a.verts[vi.z];
-        float4* t = a.tris + (size_t)(o + w) * 3;
-        t[0] = make_float4(v0.x, v0.y, v0.z, __uint_as_float((uint32_t)(vi.x / 3)));
-        t[1] = make_float4(v1.x - v0.x, v1.y - v0.y, v1.z - v0.z, __uint_as_float((uint32_t)x));
-        t[2] = make_float4(v2.x - v0.x, v2.y - v0.y, v2.z - v0.z, 0.f);
-        ++w;
-    }
+// Leaf ref of primitive n (a triangle, or an instance in the TLAS): triangle
+// n's record (v0, prim id), (v1 - v0, vert_indices slot), (v2 - v0, 0) at
+// record slot n.
+LFH_DEV uint32_t make_leaf_ref(const CollapseArgs& a, int n) {
+    if (a.tlas) return make_inst((uint32_t)n);
+    const int4 vi = a.vidx[n];
+    const float4 v0 = a.verts[vi.x], v1 = a.verts[vi.y], v2 = a.verts[vi.z];
+    float4* t = a.tris + (size_t)n * 3;
+    t[0] = make_float4(v0.x, v0.y, v0.z, __uint_as_float((uint32_t)(vi.x / 3)));
+    t[1] = make_float4(v1.x - v0.x, v1.y - v0.y, v1.z - v0.z, __uint_as_float((uint32_t)n));
+    t[2] = make_float4(v2.x - v0.x, v2.y - v0.y, v2.z - v0.z, 0.f);
+    return make_leaf((uint32_t)n, 1u);
 }
 
-// Leaf ref of a subtree small enough to be a leaf (writing its triangles).
-LFH_DEV uint32_t make_leaf_ref(const CollapseArgs& a, int n, int o) {
-    if (a.tlas) return make_inst((uint32_t)n);   // TLAS leaves hold one instance
-    write_leaf_tris(a, n, o);
-    return make_leaf((uint32_t)o, (uint32_t)a.ncount[n]);
-}
-
-LFH_DEV bool is_leaf_sized(const CollapseArgs& a, int n) { return a.ncount[n] <= a.leaf_max; }
+LFH_DEV bool is_leaf_sized(const CollapseArgs& a, int n) { return !is_internal(a, n); }
 
 // Children of a BVH4 node from binary node n (widening rule of the host path).
 LFH_DEV int expand(const CollapseArgs& a, int n, int ch[4]) {
@@ -403,7 +377,7 @@ __global__ __launch_bounds__(kBlk) void k_root_write(CollapseArgs a, const int* 
         q.need[e] = 0;
         seg_ref[m] = (uint32_t)(base4 + e);
     } else {
-        seg_ref[m] = make_leaf_ref(a, r, o);
+        seg_ref[m] = make_leaf_ref(a, r);
         atomicMax(a.need_max, 0);
     }
 }
@@ -441,9 +415,10 @@ __global__ __launch_bounds__(kBlk) void k_level_write(CollapseArgs a, Level q, i
         float4 lo = make_float4(1.f, 1.f, 1.f, 0.f), hi = make_float4(-1.f, -1.f, -1.f, 0.f);
         if (k < c) {
             const int n = ch[k];
-            const float4 l = a.nbox[(size_t)n * 2], h = a.nbox[(size_t)n * 2 + 1];
-            lo = make_float4(down(l.x), down(l.y), down(l.z), 0.f);
-            hi = make_float4(up(h.x), up(h.y), up(h.z), 0.f);
+            // exact vertex bounds, as the reference builder boxes primitives: a
+            // flat box is not entered by a ray leaving its plane at distance 0
+            lo = a.nbox[(size_t)n * 2];
+            hi = a.nbox[(size_t)n * 2 + 1];
             if (is_internal(a, n) && !is_leaf_sized(a, n)) {
                 nx.n2[slot] = n;
                 nx.n4[slot] = next_base4 + slot;
@@ -452,7 +427,7 @@ __global__ __launch_bounds__(kBlk) void k_level_write(CollapseArgs a, Level q, i
                 ref = (uint32_t)(next_base4 + slot);
                 ++slot;
             } else {
-                ref = make_leaf_ref(a, n, o);
+                ref = make_leaf_ref(a, n);
                 atomicMax(a.need_max, need);
             }
             o += a.ncount[n];
@@ -476,8 +451,8 @@ __global__ __launch_bounds__(kBlk) void k_root_boxes(const int* __restrict__ roo
     if (m >= nseg) return;
     const float4 lo = nbox[(size_t)roots[m] * 2], hi = nbox[(size_t)roots[m] * 2 + 1];
     float* o = out6 + (size_t)m * 6;
-    o[0] = down(lo.x); o[1] = down(lo.y); o[2] = down(lo.z);
-    o[3] = up(hi.x); o[4] = up(hi.y); o[5] = up(hi.z);
+    o[0] = lo.x; o[1] = lo.y; o[2] = lo.z;
+    o[3] = hi.x; o[4] = hi.y; o[5] = hi.z;
 }
 
 inline int nblk(int64_t n) { return (int)((n + kBlk - 1) / kBlk); }
@@ -685,7 +660,6 @@ std::string device_build_tlas(PackedScene* ps, DeviceScene* ds, hipStream_t s) {
     a.nkids = b.nkids.as<const int2>();
     a.ncount = b.ncount.as<const int>();
     a.nprim = n;
-    a.leaf_max = 1;
     a.tlas = 1;
     a.nodes4 = ds->nodes.as<float4>();
     std::vector<uint32_t> root;
@@ -741,7 +715,6 @@ std::string device_build(PackedScene* ps, DeviceScene* ds, hipStream_t s) {
     a.nkids = b.nkids.as<const int2>();
     a.ncount = b.ncount.as<const int>();
     a.nprim = n;
-    a.leaf_max = kLeafTris;
     a.tlas = 0;
     a.verts = dverts.as<const float4>();
     a.vidx = ds->vidx.as<const int4>();

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "ctx.h"
+#include "hdr_kernels.h"
 #include "xform.h"
 
 namespace lfh {
@@ -242,6 +243,7 @@ struct Builder {
         ps->num_tris = d.num_tris;
         ps->num_vertices = d.num_vertices;
         ps->tris.assign(tris ? (size_t)d.num_tris * 12 : 0, 0.f);
+        ps->tri_box.assign(tris ? (size_t)d.num_tris * 6 : 0, 0.f);
         ps->vidx.assign((size_t)d.num_tris * 4, 0);
         for (int64_t k = 0; k < d.num_tris; ++k) {
             int32_t vi[3] = {d.vert_indices[k * 3], d.vert_indices[k * 3 + 1], d.vert_indices[k * 3 + 2]};
@@ -251,6 +253,15 @@ struct Builder {
             ps->vidx[k * 4 + 1] = vi[1];
             ps->vidx[k * 4 + 2] = vi[2];
             if (!tris) continue;
+            {
+                float* b = &ps->tri_box[(size_t)k * 6];
+                for (int a = 0; a < 3; ++a) {
+                    const float p0 = d.vertices_uvx[(size_t)vi[0] * 4 + a], p1 = d.vertices_uvx[(size_t)vi[1] * 4 + a],
+                                p2 = d.vertices_uvx[(size_t)vi[2] * 4 + a];
+                    b[a] = std::min(std::min(p0, p1), p2);
+                    b[3 + a] = std::max(std::max(p0, p1), p2);
+                }
+            }
             const float* v0 = &d.vertices_uvx[(size_t)vi[0] * 4];
             const float* v1 = &d.vertices_uvx[(size_t)vi[1] * 4];
             const float* v2 = &d.vertices_uvx[(size_t)vi[2] * 4];
@@ -307,13 +318,17 @@ struct Builder {
         ps->hdr_w = d.hdr_width;
         ps->hdr_h = d.hdr_height;
         if (d.hdr_width > 0 && d.hdr_height > 0) {
-            if (!d.hdr_cols || !d.hdr_marginal || !d.hdr_conditional) { err = "incomplete HDR arrays"; return false; }
+            if (!d.hdr_cols) { err = "HDR size without colours"; return false; }
+            if (!d.hdr_marginal != !d.hdr_conditional) { err = "give both HDR tables or neither"; return false; }
             size_t n = (size_t)d.hdr_width * d.hdr_height;
             ps->hdr_cols.assign(n * 4, 0.f);
             for (size_t i = 0; i < n; ++i)
                 for (int c = 0; c < 3; ++c) ps->hdr_cols[i * 4 + c] = d.hdr_cols[i * 3 + c];
-            ps->hdr_marg.assign(d.hdr_marginal, d.hdr_marginal + (size_t)d.hdr_height * 2);
-            ps->hdr_cond.assign(d.hdr_conditional, d.hdr_conditional + n * 2);
+            ps->hdr_tables_on_device = !d.hdr_marginal;   // built at upload (kernels_hdr.hip)
+            if (d.hdr_marginal) {
+                ps->hdr_marg.assign(d.hdr_marginal, d.hdr_marginal + (size_t)d.hdr_height * 2);
+                ps->hdr_cond.assign(d.hdr_conditional, d.hdr_conditional + n * 2);
+            }
         }
         return true;
     }
@@ -328,17 +343,19 @@ namespace {
 // boxes, so a ray passes a collapsed child's slab test only if it passes the
 // tests of the BVH2 nodes skipped above it.  A node left with room after that
 // (all its children leaves) splits its multi-triangle leaves into one child
-// per triangle, boxed by the triangle as the traversal tests it (v0, v0 + e0,
-// v0 + e1, one ulp outward); a multi-triangle leaf that finds no such room
-// becomes a node of one-triangle children.  Fewer triangle tests, the same
-// closest hits.
+// per triangle, boxed by its vertices as the reference builder boxes
+// primitives; a multi-triangle leaf that finds no such room becomes a node of
+// one-triangle children.  Fewer triangle tests, the same closest hits: a box
+// of exact vertex bounds is entered exactly when the reference's leaf box
+// around it would lead to the triangle test (flat boxes included, which a ray
+// leaving their plane at distance 0 does not enter).
 // Nodes are allocated in preorder, so a node's first child follows it in memory.
 struct Widen {
     const std::vector<float>& n2;
     std::vector<float>& out;
     std::unordered_map<uint32_t, std::pair<uint32_t, int>> memo;  // BLAS root -> (ref4, need)
     const std::vector<float>& insts;
-    const std::vector<float>& tris;   // 12 floats per triangle slot: v0, e0, e1
+    const std::vector<float>& tbox;   // 6 floats per triangle slot: its vertices' box
 
     struct Child {
         float box[6];
@@ -356,14 +373,7 @@ struct Widen {
         std::memcpy(&c[1].ref, o + 13, 4);
     }
     static bool internal(uint32_t r) { return r != lfhd::kRefSentinel && !lfhd::ref_is_leaf(r) && !lfhd::ref_is_inst(r); }
-    void tri_box(uint32_t slot, float* b) const {
-        const float* t = &tris[(size_t)slot * 12];
-        for (int a = 0; a < 3; ++a) {
-            const float p0 = t[a], p1 = t[a] + t[4 + a], p2 = t[a] + t[8 + a];
-            b[a] = std::nextafter(std::min(std::min(p0, p1), p2), -INFINITY);
-            b[3 + a] = std::nextafter(std::max(std::max(p0, p1), p2), INFINITY);
-        }
-    }
+    void tri_box(uint32_t slot, float* b) const { std::memcpy(b, &tbox[(size_t)slot * 6], 6 * sizeof(float)); }
 
     // Returns the BVH4 ref of BVH2 ref r and, in *need, the largest number of
     // stack entries a traversal below it can hold.
@@ -474,7 +484,7 @@ struct Widen {
 
 std::string widen_bvh(PackedScene* ps) {
     ps->nodes4.clear();
-    Widen w{ps->nodes, ps->nodes4, {}, ps->insts, ps->tris};
+    Widen w{ps->nodes, ps->nodes4, {}, ps->insts, ps->tri_box};
     // BLAS trees first, in instance order, then the TLAS
     for (int i = 0; i < ps->num_instances; ++i) {
         uint32_t blas;
@@ -671,6 +681,17 @@ hipError_t upload_scene(PackedScene& ps, DeviceScene* ds, hipStream_t s, std::st
     if ((e = up(ds->hdr_cols, ps.hdr_cols, s)) != hipSuccess) return e;
     if ((e = up(ds->hdr_marg, ps.hdr_marg, s)) != hipSuccess) return e;
     if ((e = up(ds->hdr_cond, ps.hdr_cond, s)) != hipSuccess) return e;
+    if (ps.hdr_tables_on_device && ps.hdr_w > 0) {
+        const size_t n = (size_t)ps.hdr_w * ps.hdr_h;
+        DevBuf scratch;
+        if ((e = ds->hdr_marg.alloc((size_t)ps.hdr_h * 8)) != hipSuccess) return e;
+        if ((e = ds->hdr_cond.alloc(n * 8)) != hipSuccess) return e;
+        if ((e = scratch.alloc(lfhd::hdr_scratch_floats(ps.hdr_w, ps.hdr_h) * 4)) != hipSuccess) return e;
+        if ((e = lfhd::hdr_tables(ds->hdr_cols.as<const float4>(), ps.hdr_w, ps.hdr_h, scratch.as<float>(),
+                                  ds->hdr_marg.as<float2>(), ds->hdr_cond.as<float2>(), s)) != hipSuccess)
+            return e;
+        if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;   // before the scratch is freed
+    }
     if ((e = up(ds->tex, ps.tex, s)) != hipSuccess) return e;
     if ((e = up(ds->uvs, ps.uvs, s)) != hipSuccess) return e;
     if (ps.device_build) {

@@ -62,8 +62,8 @@ uint64_t hash_desc(const lfh_scene_desc& d) {
     if (d.hdr_width > 0) {
         size_t n = (size_t)d.hdr_width * d.hdr_height;
         h = fnv(h, d.hdr_cols, n * 12);
-        h = fnv(h, d.hdr_marginal, (size_t)d.hdr_height * 8);
-        h = fnv(h, d.hdr_conditional, n * 8);
+        if (d.hdr_marginal) h = fnv(h, d.hdr_marginal, (size_t)d.hdr_height * 8);
+        if (d.hdr_conditional) h = fnv(h, d.hdr_conditional, n * 8);
     }
     if (d.textures && d.num_textures > 0)
         h = fnv(h, d.textures, (size_t)d.num_textures * d.tex_width * d.tex_height * 4);
@@ -80,6 +80,7 @@ int64_t packed_bytes(const PackedScene& ps) {
                           ps.normals.size() + ps.materials.size() + ps.lights.size() + ps.hdr_cols.size() +
                           ps.hdr_marg.size() + ps.hdr_cond.size() + ps.uvs.size()) * 4 +
                 (int64_t)ps.vidx.size() * 4 + (int64_t)ps.tex.size();
+    if (ps.hdr_tables_on_device) b += (int64_t)ps.hdr_w * ps.hdr_h * 8 + (int64_t)ps.hdr_h * 8;
     if (ps.device_build)  // resident after the build: triangle records, BVH4 nodes (at most one per 3 triangles + TLAS)
         b += ps.num_tris * 48 + (ps.num_tris / 3 + 2 * (int64_t)ps.num_instances + 2) * 128;
     return b;

@@ -170,3 +170,31 @@ def test_serving_admits_a_device_built_scene():
     assert srv.status(a).state == 2 and srv.status(b).state == 2
     assert np.array_equal(ia.view(np.uint32), ib.view(np.uint32))
     assert ia.sum() > 0
+
+
+def test_environment_tables_built_on_the_device():
+    """A desc without the HDR tables gets them built on the device
+    (buildDistributions in the reference's summation order): environment
+    samples, pdfs and path-traced frames are bitwise those of the host tables."""
+    print("[dbvh] hdr tables", flush=True)
+    sc = scenes.instanced(n_meshes=4, n_instances=12, min_tris=50, max_tris=2000, width=40, height=32,
+                          hdr_size=(256, 128), seed=2)
+    host = lfh.Context.from_scene(sc)
+    d = sc.desc()
+    assert d.hdr_width == 256 and d.hdr_marginal
+    d.hdr_marginal = None
+    d.hdr_conditional = None
+    dev = lfh.Context(d, sc.options())
+    rng = np.random.default_rng(3)
+    seeds = rng.integers(0, 2**32, (100_000, 4), dtype=np.uint64).astype(np.uint32)
+    for a, b in zip(host.sample_env(seeds), dev.sample_env(seeds)):
+        assert np.array_equal(np.asarray(a).view(np.uint32), np.asarray(b).view(np.uint32))
+    dirs = rng.normal(size=(50_000, 3))
+    dirs = (dirs / np.linalg.norm(dirs, axis=1, keepdims=True)).astype(np.float32)
+    assert np.array_equal(host.env_pdf(dirs).view(np.uint32), dev.env_pdf(dirs).view(np.uint32))
+    out = []
+    for ctx in (host, dev):
+        ctx.update(sc.camera(), reset=True)
+        ctx.render_passes(3)
+        out.append(ctx.accum()[0])
+    assert np.array_equal(out[0].view(np.uint32), out[1].view(np.uint32)) and out[0].sum() > 0

@@ -99,3 +99,40 @@ def test_empty_batch_and_device_entry():
     ctx.synchronize()
     gh = d_hits.cpu().numpy().view(lfh.HIT_DTYPE)
     _compare(gh, ctx.trace_closest(rays))
+
+
+def _leaving_rays(ctx, n, seed):
+    """Continuation-like rays: from the hit points of random rays, offset
+    1e-3 along a new random direction (pathtrace.glsl:291), many of them
+    grazing the axis-aligned walls they start on."""
+    rng = np.random.default_rng(seed)
+    r = scenes.incoherent_rays(n, seed=seed)
+    r["org"] = rng.random((n, 3)).astype(np.float32)
+    h = ctx.trace_closest(r)
+    hit = h["prim"] >= 0
+    p = r["org"][hit] + r["dir"][hit] * h["t"][hit][:, None]
+    m = int(hit.sum())
+    d = rng.normal(size=(m, 3))
+    d[: m // 2, 1] *= 1e-4          # half of them nearly parallel to the floor / ceiling
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    out = scenes.incoherent_rays(m, seed=seed + 1)
+    out["dir"] = d
+    out["org"] = (p + d * 1e-3).astype(np.float32)
+    return out
+
+
+@pytest.mark.parametrize("device_bvh", [False, True])
+def test_rays_leaving_flat_walls_match_oracle(device_bvh):
+    """A ray that leaves the plane of an axis-aligned wall at distance 0 does
+    not enter that wall's flat box, so it cannot hit the wall at t = -0 (the
+    reference's AABBIntersect > 0 rule): the BVH4 leaves split into
+    one-triangle children and the device-built tree box triangles by their
+    exact vertex bounds, and hit what the reference traversal hits."""
+    print(f"[trace] leaving rays device_bvh={device_bvh}", flush=True)
+    sc = scenes.cornell()
+    ctx = lfh.Context.from_scene(sc, device_bvh=device_bvh)
+    orc = native.OracleScene(sc.arrays())
+    rays = _leaving_rays(ctx, 600_000, seed=11)
+    gh = ctx.trace_closest(rays)
+    _compare(gh, orc.trace_closest(rays, tie_mode=1))
+    assert not (gh["t"][gh["prim"] >= 0] <= 0).any()
