@@ -35,8 +35,14 @@ namespace {
 constexpr int kBatch = 65536;             // samples per optimisation step
 constexpr int kMaxSamples = LFH_GUIDE_SAMPLE_BATCHES * kBatch;  // samples kept from one wavefront
 constexpr int kRecordedPaths = 1 << LFH_GUIDE_RECORD_LOG2;     // recorded paths per wavefront, at most
-constexpr int kWarmSteps = 256;           // Adam steps taken at the warm-up rate
-constexpr int kWarmRepeats = 4;           // optimisation passes over a pass's samples while warming up
+#ifndef LFH_GUIDE_WARM_STEPS
+#define LFH_GUIDE_WARM_STEPS 128   // 64: 0.565, 128: 0.564, 256: 0.604 guided/plain time on the guiding room
+#endif
+#ifndef LFH_GUIDE_WARM_REPEATS
+#define LFH_GUIDE_WARM_REPEATS 4
+#endif
+constexpr int kWarmSteps = LFH_GUIDE_WARM_STEPS;       // Adam steps taken at the warm-up rate
+constexpr int kWarmRepeats = LFH_GUIDE_WARM_REPEATS;   // optimisation passes over a pass's samples while warming up
 constexpr float kWeightCap = 32.0f;       // online training: weights / their mean, capped
 }  // namespace
 
