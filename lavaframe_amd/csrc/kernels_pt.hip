@@ -77,7 +77,7 @@ __global__ __launch_bounds__(256) void k_generate(CamParams cam1, const JobDev* 
         pb.job[p] = j;
     }
     const int P = cam.tile_w * cam.tile_h;
-    if (!MULTI && cam.batch > 1 && inrange) {
+    if (cam.batch > 1 && inrange) {
         const int k = lp / P;   // sample k of the batch: the frame after k passes
         lp -= k * P;
         cam.frame += k;
@@ -542,7 +542,7 @@ __global__ __launch_bounds__(256) void k_accumulate(PathBufs pb, int total, CamP
     float4 s = accum[pix];
     float4 c = kahan ? comp[pix] : make_float4(0.f, 0.f, 0.f, 0.f);
     // the batch's samples of this pixel, added in frame order as separate passes would
-    const int nb = MULTI ? 1 : cam.batch;
+    const int nb = cam.batch > 1 ? cam.batch : 1;
     const int P = cam.tile_w * cam.tile_h;
     for (int k = 0; k < nb; ++k) {
         const int q = p + k * P;

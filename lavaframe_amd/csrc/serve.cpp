@@ -39,6 +39,8 @@ namespace {
 // Device bytes of one pass's path state + queues per path (PassBufs).
 constexpr int64_t kPathBytes = 16 * (1 + 1 + 1 + 1 + 1 + 3 + 2 + 2 + 2 + 4) + 4;
 constexpr int64_t kAccumBytes = 32;  // sum + compensation, float4 each
+constexpr int64_t kStepPaths = 1 << 23;   // paths per server step when the budget allows
+constexpr int64_t kStepMaxPasses = 64;    // passes of one job per step, at most
 
 uint64_t fnv(uint64_t h, const void* p, size_t n) {
     const unsigned char* c = static_cast<const unsigned char*>(p);
@@ -293,9 +295,34 @@ int lfh_server_step(lfh_server* s, int32_t* active) {
         j->t_admit = s->now();
         running.push_back(j);
     }
-    // -- one pass for every running job, rays packed into shared queues
+    // -- passes for every running job, rays packed into shared queues: one pass
+    //    each; then the jobs whose remaining work fits in the rest of the
+    //    step's path budget finish in this step, least work first; what is
+    //    left goes to the oldest jobs first (several passes of a job are
+    //    accumulated per pixel in frame order: the image is that of single passes)
     if (!running.empty()) {
         std::sort(running.begin(), running.end(), [](const lfh::Job* a, const lfh::Job* b) { return a->id < b->id; });
+        int64_t one_pass = 0;
+        for (lfh::Job* j : running) one_pass += j->npix;
+        const int64_t spare = std::max<int64_t>(0, (s->budget - s->used) / lfh::kPathBytes);
+        int64_t left_paths = std::max<int64_t>(0, std::min(lfh::kStepPaths, one_pass + spare) - one_pass);
+        std::vector<int> passes(running.size(), 1);
+        std::vector<size_t> order(running.size());
+        for (size_t k = 0; k < order.size(); ++k) order[k] = k;
+        auto work_left = [&](size_t k) { return ((int64_t)running[k]->target - running[k]->spp_done) * running[k]->npix; };
+        std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return work_left(a) < work_left(b); });
+        auto give = [&](size_t k, bool only_if_done) {
+            lfh::Job* j = running[k];
+            const int64_t left = (int64_t)j->target - j->spp_done;
+            const int64_t want = std::min<int64_t>(left, lfh::kStepMaxPasses) - passes[k];
+            if (want <= 0) return;
+            if (only_if_done && want * j->npix > left_paths) return;
+            const int64_t extra = std::min<int64_t>(want, left_paths / j->npix);
+            passes[k] += (int)extra;
+            left_paths -= extra * j->npix;
+        };
+        for (size_t k : order) give(k, true);                  // finish what fits, least work first
+        for (size_t k = 0; k < running.size(); ++k) give(k, false);   // then by arrival (running is id-sorted)
         int64_t total = 0;
         int max_depth = 0;
         std::vector<lfhd::JobDev> tab(running.size());
@@ -319,18 +346,21 @@ int lfh_server_step(lfh_server* s, int32_t* active) {
             c.tileX = 0;
             c.tileY = 0;
             c.frame = j->frame;
+            c.batch = passes[k];
             d.accum = j->accum.as<float4>();
             d.comp = j->comp.as<float4>();
             d.path_offset = (int)total;
             d.kahan = j->opt.kahan;
-            total += j->npix;
+            total += j->npix * passes[k];
             max_depth = std::max(max_depth, j->opt.max_depth);
             j->scene->last_use = s->now();
         }
         if (total > (1LL << 30)) return sfail(s, LFH_E_UNSUPPORTED, "lfh_server_step: too many pixels in flight");
-        if (s->pass.cap < total || s->pass.cap > 2 * total + (1 << 20)) {
+        if (s->pass.cap < total || s->pass.cap > 4 * total + (1 << 23)) {
+            // grow with headroom, shrink only when far too large (a reallocation waits for the GPU)
+            const int64_t cap = std::min<int64_t>(total + total / 4, std::max<int64_t>(total, lfh::kStepPaths));
             SCHK(s, hipStreamSynchronize(s->stream), "lfh_server_step: sync");
-            SCHK(s, s->pass.alloc((int)total), "lfh_server_step: path buffers");
+            SCHK(s, s->pass.alloc((int)cap), "lfh_server_step: path buffers");
         }
         SCHK(s, hipMemcpyAsync(s->table.p, tab.data(), tab.size() * sizeof(lfhd::JobDev), hipMemcpyHostToDevice,
                                s->stream),
@@ -355,9 +385,10 @@ int lfh_server_step(lfh_server* s, int32_t* active) {
         SCHK(s, pt_accumulate_multi(jt, (int)total, v, s->stream), "lfh_server_step: accumulate");
         SCHK(s, hipStreamSynchronize(s->stream), "lfh_server_step: sync");
         double t = s->now();
-        for (lfh::Job* j : running) {
-            j->spp_done++;
-            j->frame++;
+        for (size_t k = 0; k < running.size(); ++k) {
+            lfh::Job* j = running[k];
+            j->spp_done += passes[k];
+            j->frame += passes[k];
             if (j->spp_done >= j->target) {
                 j->state = 2;
                 j->t_done = t;
