@@ -56,6 +56,16 @@ LFH_DEV void mlp_forward_rows(const GuideDev& g, uint16_t* X, float* O, int ntil
         const float* bias = g.bias + layer * 64;
         constexpr int kN[4] = {4, 4, 4, 3};
         const int nN = kN[layer];
+        // the layer's B fragments and biases once per wave, reused by every row tile
+        bf16x8 bw[4][2];
+        float bb[4];
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) {
+            if (nt >= nN) continue;
+            bb[nt] = bias[nt * 16 + lr];
+#pragma unroll
+            for (int ks = 0; ks < 2; ++ks) bw[nt][ks] = ld_frag(W + (nt * 16 + lr) * 64 + ks * 32 + lk);
+        }
         for (int mt = 0; mt < ntiles; ++mt) {
             f32x4 acc[4];
 #pragma unroll
@@ -65,13 +75,13 @@ LFH_DEV void mlp_forward_rows(const GuideDev& g, uint16_t* X, float* O, int ntil
                 const bf16x8 a = ld_frag(X + (mt * 16 + lr) * kPitch + ks * 32 + lk);
 #pragma unroll
                 for (int nt = 0; nt < 4; ++nt)
-                    if (nt < nN) acc[nt] = mfma(a, ld_frag(W + (nt * 16 + lr) * 64 + ks * 32 + lk), acc[nt]);
+                    if (nt < nN) acc[nt] = mfma(a, bw[nt][ks], acc[nt]);
             }
 #pragma unroll
             for (int nt = 0; nt < 4; ++nt) {
                 if (nt >= nN) continue;
                 const int col = nt * 16 + lr;
-                const float b = bias[col];
+                const float b = bb[nt];
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
                     const int row = mt * 16 + lq + j;
