@@ -187,7 +187,7 @@ def main():
         legs = {"c0": cornell_leg, "c2": instanced_leg, "c4": serving_leg}
         for leg in args.legs.split(","):
             try:
-                extra.update(legs[leg.strip()](stream))
+                extra.update(legs[leg.strip()](stream, cpu=not args.no_cpu_baseline))
             except Exception as e:  # a failing extra leg must not hide the main measurement
                 log(f"leg {leg} failed: {type(e).__name__}: {e}")
                 extra[f"{leg}_error"] = f"{type(e).__name__}: {e}"
@@ -317,6 +317,43 @@ def cpu_baseline(scene, rays, n_coh, sample):
 
 
 # ---- path-tracing legs ------------------------------------------------------------------
+def pt_cpu_baseline(sc, spp, seconds=8.0):
+    """The oracle's FP32 restatement of the path tracer (pathtrace.glsl with the
+    reference's RNG and BVH visit order) on all host cores, over an evenly
+    strided subset of the frame's pixels, sized from a short calibration run to
+    take about `seconds`.  Returns the rate in Msamples/s and the sample."""
+    try:
+        from oracle import native
+    except Exception as e:  # pragma: no cover
+        log(f"path-tracing cpu baseline unavailable: {e}")
+        return None
+    if not native.available():
+        log("path-tracing cpu baseline unavailable: oracle not built")
+        return None
+    arr = sc.arrays()
+    opts = sc.options()
+    o = native.options_from(opts, sc.camera(), lights=len(arr["lights"]) > 0, envmap=bool(opts.use_env_map))
+    orc = native.OracleScene(arr)
+    threads = native.threads()
+    w, h = opts.width, opts.height
+
+    def run(n_px):
+        px = np.linspace(0, w * h - 1, n_px).astype(np.int32)
+        t = time.perf_counter()
+        orc.render_pixels(o, px, 1, spp, precision=32, nthreads=threads)
+        return time.perf_counter() - t
+
+    n = 64 * threads
+    dt = run(n)
+    n = int(min(w * h, max(n, n * seconds / max(dt, 1e-3))))
+    dt = run(n)
+    samples = n * spp
+    log(f"path-tracing cpu baseline: {n} pixels x {spp} spp in {dt:.2f}s on {threads} threads")
+    return {"value": round(samples / dt / 1e6, 4), "unit": "Msamples/s", "cores": threads, "kind": "port",
+            "sample": f"{n} of {w * h} pixels (evenly strided) x {spp} spp of the same frame; FP32 restatement "
+                      "of the reference's path tracer"}
+
+
 def _timed_passes(ctx, stream, passes):
     """Renders `passes` full-frame passes; returns GPU milliseconds (HIP events
     on the context's stream)."""
@@ -330,7 +367,7 @@ def _timed_passes(ctx, stream, passes):
     return e0.elapsed_time(e1)
 
 
-def cornell_leg(stream):
+def cornell_leg(stream, cpu=True):
     """Cornell box 256 x 256, 64 samples per pixel, max depth 4 (one tile)."""
     from lavaframe_amd import lfh, scenes
     sc = scenes.cornell(256, 256, max_depth=4)
@@ -348,12 +385,15 @@ def cornell_leg(stream):
     samples = 256 * 256 * spp
     rays = (r1[0] - r0[0]) + (r1[1] - r0[1])
     log(f"cornell 256^2 x {spp} spp: {ms:.2f} ms, {samples / ms / 1e3:.1f} Msamples/s, {rays / ms / 1e3:.1f} Mrays/s")
-    return {"cornell_c0": {"msamples_per_s": round(samples / (ms * 1e-3) / 1e6, 1),
-                           "mrays_per_s": round(rays / (ms * 1e-3) / 1e6, 1), "spp": spp,
-                           "resolution": [256, 256], "max_depth": 4, "ms": round(ms, 2)}}
+    out = {"msamples_per_s": round(samples / (ms * 1e-3) / 1e6, 1),
+           "mrays_per_s": round(rays / (ms * 1e-3) / 1e6, 1), "spp": spp,
+           "resolution": [256, 256], "max_depth": 4, "ms": round(ms, 2)}
+    if cpu:
+        out["cpu_baseline"] = pt_cpu_baseline(sc, spp)
+    return {"cornell_c0": out}
 
 
-def instanced_leg(stream):
+def instanced_leg(stream, cpu=True):
     """64 procedural meshes (1k-200k triangles), 1024 random affine instances,
     random Disney materials, 8 quad lights + 2048 x 1024 sky; 1920 x 1080,
     max depth 8; 16 samples per pixel timed."""
@@ -375,10 +415,14 @@ def instanced_leg(stream):
     ext, sh = r1[0] - r0[0], r1[1] - r0[1]
     log(f"instanced 1920x1080 x {spp} spp: {ms:.1f} ms, {samples / ms / 1e3:.1f} Msamples/s, "
         f"{(ext + sh) / ms / 1e3:.1f} Mrays/s")
-    return {"instanced_c2": {"msamples_per_s": round(samples / (ms * 1e-3) / 1e6, 1),
-                             "mrays_per_s": round((ext + sh) / (ms * 1e-3) / 1e6, 1),
-                             "extend_rays": ext, "shadow_rays": sh, "spp": spp, "triangles": ntri,
-                             "instances": 1024, "resolution": [1920, 1080], "max_depth": 8, "ms": round(ms, 1)}}
+    out = {"msamples_per_s": round(samples / (ms * 1e-3) / 1e6, 1),
+           "mrays_per_s": round((ext + sh) / (ms * 1e-3) / 1e6, 1),
+           "extend_rays": ext, "shadow_rays": sh, "spp": spp, "triangles": ntri,
+           "instances": 1024, "resolution": [1920, 1080], "max_depth": 8, "ms": round(ms, 1)}
+    if cpu:
+        del ctx
+        out["cpu_baseline"] = pt_cpu_baseline(sc, 1)
+    return {"instanced_c2": out}
 
 
 def _job_stream(n_jobs, rate, seed):
@@ -431,11 +475,11 @@ def _run_stream(scn, jobs, max_jobs):
             "aggregate_msamples_per_s": round(samples / total / 1e6, 1), "makespan_s": round(total, 3)}
 
 
-def serving_leg(stream):
+def serving_leg(stream, cpu=True):
     """Render serving: 60 jobs, Poisson arrivals at 20 jobs/s, from the Cornell,
     instanced (small) and guiding-room generators at 256^2-960x540 and 16-128
     spp; latency = submit to target spp.  Baseline: the same stream with one
-    job admitted at a time."""
+    job admitted at a time (so `cpu` is not used here)."""
     from lavaframe_amd import scenes
     scn = [scenes.cornell(256, 256), scenes.instanced(n_meshes=16, n_instances=128, max_tris=50000,
                                                       hdr_size=(512, 256)),

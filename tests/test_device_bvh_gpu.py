@@ -198,3 +198,21 @@ def test_environment_tables_built_on_the_device():
         ctx.render_passes(3)
         out.append(ctx.accum()[0])
     assert np.array_equal(out[0].view(np.uint32), out[1].view(np.uint32)) and out[0].sum() > 0
+
+
+def test_guided_render_on_a_device_built_scene_equals_host_built():
+    """The guide queries the hit triangle's record by its vert_indices slot
+    (geometric normal): a device-built scene keeps records in that order, so
+    online-trained guided frames are bitwise those of the host-built scene."""
+    print("[dbvh] guided", flush=True)
+    sc = scenes.aperture_room(64, 48, target_tris=20000)
+    out = []
+    for dev in (False, True):
+        ctx = lfh.Context.from_scene(sc, device_bvh=dev, estimator=1, guiding=1)
+        g = lfh.Guide.for_scene(sc, seed=4)
+        ctx.set_guide(g, train_every=1)
+        ctx.update(sc.camera(), reset=True)
+        ctx.render_passes(12)
+        out.append((ctx.accum()[0], g.params()))
+    assert np.array_equal(out[0][0].view(np.uint32), out[1][0].view(np.uint32))
+    assert np.array_equal(out[0][1].view(np.uint32), out[1][1].view(np.uint32))
