@@ -52,6 +52,9 @@ LFH_HD uint32_t make_inst(uint32_t inst) { return kRefInstBit | inst; }
 #ifndef LFH_LDS_STACK
 #define LFH_LDS_STACK 16
 #endif
+#ifndef LFH_TRACE_WAVES
+#define LFH_TRACE_WAVES 6   // waves per SIMD the ray-batch traversal kernel is register-budgeted for
+#endif
 #ifndef LFH_POP_CULL
 #define LFH_POP_CULL 0   // 1: keep box entry distances on the stack and drop stale entries at pop
 #endif

@@ -16,7 +16,7 @@ __global__ void k_zero_u64(unsigned long long* p, int n) {
 }
 
 template <bool ANY, bool STATS>
-__global__ __launch_bounds__(kTraceBlock) void k_trace(TraceScene sc, const float4* __restrict__ rays, int64_t n,
+__global__ __launch_bounds__(kTraceBlock, LFH_TRACE_WAVES) void k_trace(TraceScene sc, const float4* __restrict__ rays, int64_t n,
                                                        int32_t* __restrict__ hits, uint8_t* __restrict__ occl,
                                                        int32_t* __restrict__ counts, uint32_t* __restrict__ spill,
                                                        unsigned long long* __restrict__ work, float cull,
