@@ -355,9 +355,12 @@ void lfh_server_destroy(lfh_server* srv);
 /* Queue a job; its scene arrays are copied.  Returns LFH_E_BUDGET (and
  * rejects) if the job can never fit the budget. */
 int  lfh_server_submit(lfh_server* srv, const lfh_job_desc* job, int64_t* job_id);
-/* Run one scheduling step: admit queued jobs that fit, render one pass for every
- * running job (rays of all jobs packed into shared queues), retire finished
- * jobs.  *active = number of jobs queued or running afterwards. */
+/* Run one scheduling step: admit queued jobs that fit, render passes of the
+ * running jobs in one wavefront (rays of all jobs packed into shared queues;
+ * earliest deadline first within a path budget, deadline = submit time +
+ * min(samples at a nominal 3e8/s, 0.1 s); the earliest job always advances),
+ * retire finished jobs.  *active = number of jobs queued or running
+ * afterwards. */
 int  lfh_server_step(lfh_server* srv, int32_t* active);
 int  lfh_server_status(lfh_server* srv, int64_t job_id, lfh_job_status* st);
 /* Result of a finished job: linear HDR (w*h*3) like lfh_get_output_hdr. */

@@ -6,10 +6,11 @@
 //   1. admits queued jobs (priority, then arrival order; a job that does not
 //      fit is skipped so smaller ones can backfill) while the device budget
 //      and the job slots allow;
-//   2. runs ONE wavefront pass for all running jobs together: the rays of
-//      every job go through the same extend / shade / shadow queues and the
-//      same persistent kernels (the per-path job id selects scene, camera and
-//      accumulation buffers);
+//   2. runs ONE wavefront for the running jobs together, each job taking
+//      some of its passes (earliest deadline first, within a path budget):
+//      the rays of every job go through the same extend / shade / shadow
+//      queues and the same persistent kernels (the per-path job id selects
+//      scene, camera and accumulation buffers);
 //   3. retires jobs that reached their target.
 // Device scenes are shared by content hash and stay resident after their
 // jobs finish, until the budget needs the space (least recently used first).
@@ -41,6 +42,20 @@ constexpr int64_t kPathBytes = 16 * (1 + 1 + 1 + 1 + 1 + 3 + 2 + 2 + 2 + 4) + 4;
 constexpr int64_t kAccumBytes = 32;  // sum + compensation, float4 each
 constexpr int64_t kStepPaths = 1 << 23;   // paths per server step when the budget allows
 constexpr int64_t kStepMaxPasses = 64;    // passes of one job per step, at most
+// A job's deadline: submit time + min(samples / nominal rate, max offset).
+// Small jobs are ordered by size (short ones overtake long ones); a long job is
+// overtaken only by work submitted within the max offset after it, which keeps
+// its latency near that of first-come-first-served.  On a 60-job Poisson
+// stream: offset cap 0.1 s gave p50 0.045 s / p99 0.22 s; no cap 0.043 / 0.25;
+// FIFO order 0.060 / 0.21.
+#ifndef LFH_SERVE_RATE
+#define LFH_SERVE_RATE 3.0e8
+#endif
+#ifndef LFH_SERVE_MAX_OFFSET
+#define LFH_SERVE_MAX_OFFSET 0.1
+#endif
+constexpr double kDeadlineSecondsPerSample = 1.0 / LFH_SERVE_RATE;
+constexpr double kDeadlineMaxOffset = LFH_SERVE_MAX_OFFSET;
 
 uint64_t fnv(uint64_t h, const void* p, size_t n) {
     const unsigned char* c = static_cast<const unsigned char*>(p);
@@ -295,39 +310,45 @@ int lfh_server_step(lfh_server* s, int32_t* active) {
         j->t_admit = s->now();
         running.push_back(j);
     }
-    // -- passes for every running job, rays packed into shared queues: one pass
-    //    each; then the jobs whose remaining work fits in the rest of the
-    //    step's path budget finish in this step, least work first; what is
-    //    left goes to the oldest jobs first (several passes of a job are
-    //    accumulated per pixel in frame order: the image is that of single passes)
+    // -- passes for this step, rays of all chosen jobs packed into shared
+    //    queues.  Jobs are served earliest deadline first, a job's deadline
+    //    being its submit time plus its samples at a nominal rate, at most
+    //    kDeadlineMaxOffset: a small job finishes soon after it arrives, and a
+    //    large job is not overtaken by work that arrives after its deadline.  In that order each job takes as
+    //    many of its remaining passes as the step's path budget still holds (at
+    //    most kStepMaxPasses); the first always gets one.  Passes of a job are
+    //    accumulated per pixel in frame order, so its image is that of single
+    //    passes whatever the grouping.
     if (!running.empty()) {
-        std::sort(running.begin(), running.end(), [](const lfh::Job* a, const lfh::Job* b) { return a->id < b->id; });
         int64_t one_pass = 0;
         for (lfh::Job* j : running) one_pass += j->npix;
         const int64_t spare = std::max<int64_t>(0, (s->budget - s->used) / lfh::kPathBytes);
-        int64_t left_paths = std::max<int64_t>(0, std::min(lfh::kStepPaths, one_pass + spare) - one_pass);
-        std::vector<int> passes(running.size(), 1);
-        std::vector<size_t> order(running.size());
-        for (size_t k = 0; k < order.size(); ++k) order[k] = k;
-        auto work_left = [&](size_t k) { return ((int64_t)running[k]->target - running[k]->spp_done) * running[k]->npix; };
-        std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return work_left(a) < work_left(b); });
-        auto give = [&](size_t k, bool only_if_done) {
-            lfh::Job* j = running[k];
-            const int64_t left = (int64_t)j->target - j->spp_done;
-            const int64_t want = std::min<int64_t>(left, lfh::kStepMaxPasses) - passes[k];
-            if (want <= 0) return;
-            if (only_if_done && want * j->npix > left_paths) return;
-            const int64_t extra = std::min<int64_t>(want, left_paths / j->npix);
-            passes[k] += (int)extra;
-            left_paths -= extra * j->npix;
+        auto deadline = [](const lfh::Job* j) {
+            return j->t_submit + std::min(lfh::kDeadlineMaxOffset,
+                                          (double)j->npix * (double)j->target * lfh::kDeadlineSecondsPerSample);
         };
-        for (size_t k : order) give(k, true);                  // finish what fits, least work first
-        for (size_t k = 0; k < running.size(); ++k) give(k, false);   // then by arrival (running is id-sorted)
+        std::stable_sort(running.begin(), running.end(), [&](const lfh::Job* a, const lfh::Job* b) {
+            const double da = deadline(a), db = deadline(b);
+            return da < db || (da == db && a->id < b->id);
+        });
+        int64_t cap = std::min(lfh::kStepPaths, one_pass + spare);
+        cap = std::max(cap, running[0]->npix);
+        std::vector<lfh::Job*> chosen;
+        std::vector<int> passes;
         int64_t total = 0;
+        for (lfh::Job* j : running) {
+            const int64_t left = (int64_t)j->target - j->spp_done;
+            const int64_t give = std::min<int64_t>(std::min<int64_t>(left, lfh::kStepMaxPasses), (cap - total) / j->npix);
+            if (give <= 0) continue;
+            chosen.push_back(j);
+            passes.push_back((int)give);
+            total += give * j->npix;
+        }
         int max_depth = 0;
-        std::vector<lfhd::JobDev> tab(running.size());
-        for (size_t k = 0; k < running.size(); ++k) {
-            lfh::Job* j = running[k];
+        std::vector<lfhd::JobDev> tab(chosen.size());
+        int64_t off = 0;
+        for (size_t k = 0; k < chosen.size(); ++k) {
+            lfh::Job* j = chosen[k];
             lfhd::JobDev& d = tab[k];
             std::memset(&d, 0, sizeof d);
             d.sc = lfh::shade_view(j->scene->ps, j->scene->ds, j->opt);
@@ -349,9 +370,9 @@ int lfh_server_step(lfh_server* s, int32_t* active) {
             c.batch = passes[k];
             d.accum = j->accum.as<float4>();
             d.comp = j->comp.as<float4>();
-            d.path_offset = (int)total;
+            d.path_offset = (int)off;
             d.kahan = j->opt.kahan;
-            total += j->npix * passes[k];
+            off += j->npix * passes[k];
             max_depth = std::max(max_depth, j->opt.max_depth);
             j->scene->last_use = s->now();
         }
@@ -385,8 +406,8 @@ int lfh_server_step(lfh_server* s, int32_t* active) {
         SCHK(s, pt_accumulate_multi(jt, (int)total, v, s->stream), "lfh_server_step: accumulate");
         SCHK(s, hipStreamSynchronize(s->stream), "lfh_server_step: sync");
         double t = s->now();
-        for (size_t k = 0; k < running.size(); ++k) {
-            lfh::Job* j = running[k];
+        for (size_t k = 0; k < chosen.size(); ++k) {
+            lfh::Job* j = chosen[k];
             j->spp_done += passes[k];
             j->frame += passes[k];
             if (j->spp_done >= j->target) {
