@@ -354,6 +354,28 @@ def pt_cpu_baseline(sc, spp, seconds=8.0):
                       "of the reference's path tracer"}
 
 
+def frame_error(ctx, spp, ref_name):
+    """RMSE of the context's accumulated frame (spp samples per pixel) against
+    the FP64 oracle's 65,536-spp reference in tests/golden (every pixel, or the
+    stored pixel subset), beside the RMSE the two sample counts' noise alone
+    predicts from the reference's per-sample deviation."""
+    f = ROOT / "tests" / "golden" / ref_name
+    if not f.exists():
+        return None
+    z = np.load(f)
+    mean = z["mean"].reshape(-1, 3).astype(np.float64)
+    std = z["std"].reshape(-1, 3).astype(np.float64)
+    s, c = ctx.accum()
+    img = ((s.astype(np.float64) - c) / spp)[..., :3].reshape(-1, 3)
+    if "pixels" in z.files:
+        img = img[z["pixels"]]
+    rmse = float(np.sqrt(np.mean((img - mean) ** 2)))
+    noise = float(np.sqrt(np.mean(std ** 2) * (1.0 / spp + 1.0 / int(z["spp"]))))
+    return {"pixels": len(mean), "spp": spp, "reference_spp": int(z["spp"]), "rmse": round(rmse, 6),
+            "rmse_from_noise": round(noise, 6), "relative_rmse": round(rmse / float(mean.mean()), 5),
+            "mean": round(float(img.mean()), 6), "reference_mean": round(float(mean.mean()), 6)}
+
+
 def _timed_passes(ctx, stream, passes):
     """Renders `passes` full-frame passes; returns GPU milliseconds (HIP events
     on the context's stream)."""
@@ -388,6 +410,7 @@ def cornell_leg(stream, cpu=True):
     out = {"msamples_per_s": round(samples / (ms * 1e-3) / 1e6, 1),
            "mrays_per_s": round(rays / (ms * 1e-3) / 1e6, 1), "spp": spp,
            "resolution": [256, 256], "max_depth": 4, "ms": round(ms, 2)}
+    out["error_vs_fp64_64k"] = frame_error(ctx, spp, "cornell_c0_fp64_64k.npz")
     if cpu:
         out["cpu_baseline"] = pt_cpu_baseline(sc, spp)
     return {"cornell_c0": out}
@@ -407,6 +430,7 @@ def instanced_leg(stream, cpu=True):
     cam = sc.camera()
     ctx.update(cam)
     _timed_passes(ctx, stream, 2)
+    ctx.update(cam, reset=True)
     r0 = ctx.ray_counts()
     spp = 16
     ms = _timed_passes(ctx, stream, spp)
@@ -419,6 +443,7 @@ def instanced_leg(stream, cpu=True):
            "mrays_per_s": round((ext + sh) / (ms * 1e-3) / 1e6, 1),
            "extend_rays": ext, "shadow_rays": sh, "spp": spp, "triangles": ntri,
            "instances": 1024, "resolution": [1920, 1080], "max_depth": 8, "ms": round(ms, 1)}
+    out["error_vs_fp64_64k"] = frame_error(ctx, spp, "instanced_c2_fp64_64k.npz")
     if cpu:
         del ctx
         out["cpu_baseline"] = pt_cpu_baseline(sc, 1)

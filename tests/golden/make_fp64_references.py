@@ -1,0 +1,79 @@
+"""Writes the FP64 oracle references the C0 and C2 error figures are measured
+against (test infrastructure; about an hour on 8 host cores).
+
+* cornell_c0_fp64_64k.npz: the Cornell box, 256 x 256, depth 4, every pixel at
+  65,536 samples per pixel (frames 1,000,001 ... so that the samples share no
+  random numbers with a GPU render, which uses frames 1 ...).
+* instanced_c2_fp64_64k.npz: the instanced scene (64 meshes, 1024 instances,
+  1920 x 1080, depth 8), one pixel drawn in each cell of a 32 x 32 grid over the
+  frame (seeded), each at 65,536 samples per pixel.
+
+Each file holds the per-pixel mean radiance (float32) and the per-sample
+standard deviation (float16), which tests and the benchmark use to tell noise
+from bias.  Run from the repository root: python tests/golden/make_fp64_references.py
+"""
+from __future__ import annotations
+
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[2]
+sys.path.insert(0, str(ROOT))
+
+from lavaframe_amd import scenes  # noqa: E402
+from oracle import native  # noqa: E402
+
+OUT = Path(__file__).resolve().parent
+SPP = 65536
+FRAME0 = 1_000_001
+
+
+def c2_pixels(width=1920, height=1080, grid=32, seed=5):
+    """One pixel per cell of a grid x grid partition of the frame."""
+    rng = np.random.default_rng(seed)
+    px = []
+    for gy in range(grid):
+        y0, y1 = gy * height // grid, (gy + 1) * height // grid
+        for gx in range(grid):
+            x0, x1 = gx * width // grid, (gx + 1) * width // grid
+            px.append(int(rng.integers(y0, y1)) * width + int(rng.integers(x0, x1)))
+    return np.array(px, np.int32)
+
+
+def render(sc, px, chunk, label):
+    arr = sc.arrays()
+    opts = sc.options()
+    o = native.options_from(opts, sc.camera(), lights=len(arr["lights"]) > 0, envmap=bool(opts.use_env_map))
+    orc = native.OracleScene(arr)
+    mean = np.zeros((len(px), 3), np.float64)
+    std = np.zeros((len(px), 3), np.float64)
+    t0 = time.time()
+    for a in range(0, len(px), chunk):
+        sl = slice(a, min(a + chunk, len(px)))
+        s1, s2 = orc.render_pixels(o, px[sl], FRAME0, SPP, precision=64)
+        m = s1 / SPP
+        mean[sl] = m
+        std[sl] = np.sqrt(np.maximum(s2 / SPP - m * m, 0.0))
+        print(f"[{label}] {sl.stop}/{len(px)} pixels, {time.time() - t0:.0f} s", flush=True)
+    return mean, std
+
+
+def main():
+    which = sys.argv[1:] or ["c2", "c0"]
+    if "c2" in which:
+        px = c2_pixels()
+        mean, std = render(scenes.instanced(), px, 64, "c2")
+        np.savez_compressed(OUT / "instanced_c2_fp64_64k.npz", pixels=px, mean=mean.astype(np.float32),
+                            std=std.astype(np.float16), spp=SPP, frame0=FRAME0)
+    if "c0" in which:
+        px = np.arange(256 * 256, dtype=np.int32)
+        mean, std = render(scenes.cornell(256, 256, max_depth=4), px, 4096, "c0")
+        np.savez_compressed(OUT / "cornell_c0_fp64_64k.npz", mean=mean.astype(np.float32).reshape(256, 256, 3),
+                            std=std.astype(np.float16).reshape(256, 256, 3), spp=SPP, frame0=FRAME0)
+
+
+if __name__ == "__main__":
+    main()
