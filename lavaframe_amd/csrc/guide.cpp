@@ -41,7 +41,11 @@ constexpr int kRecordedPaths = 1 << LFH_GUIDE_RECORD_LOG2;     // recorded paths
 #ifndef LFH_GUIDE_WARM_REPEATS
 #define LFH_GUIDE_WARM_REPEATS 4
 #endif
+#ifndef LFH_GUIDE_LATE_REPEATS
+#define LFH_GUIDE_LATE_REPEATS 1   // 1: 0.564, 2: 0.638, 4: 0.611 guided/plain time on the guiding room
+#endif
 constexpr int kWarmSteps = LFH_GUIDE_WARM_STEPS;       // Adam steps taken at the warm-up rate
+constexpr int kLateRepeats = LFH_GUIDE_LATE_REPEATS;   // Adam steps per training pass after the warm-up
 constexpr int kWarmRepeats = LFH_GUIDE_WARM_REPEATS;   // optimisation passes over a pass's samples while warming up
 constexpr float kWeightCap = 32.0f;       // online training: weights / their mean, capped
 }  // namespace
@@ -279,7 +283,7 @@ int guide_pass_end(lfh_ctx* c, const GuideHook& gh, int batch) {
     // one Adam step per training pass on a strided batch of the wavefront's
     // samples; several while the guide is young
     for (int t = 0; t < ntrain; ++t) {
-        const int reps = g->steps < kWarmSteps ? kWarmRepeats : 1;
+        const int reps = g->steps < kWarmSteps ? kWarmRepeats : kLateRepeats;
         for (int r = 0; r < reps; ++r) {
             lfhd::GuideAdam ad = g->adam(true);
             e = lfhd::guide_train(g->dev(), b.samples.as<float4>(), b.total.as<int>(), (int)(g->steps & 0x7fffffff),

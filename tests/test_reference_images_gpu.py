@@ -21,11 +21,14 @@ GOLD = Path(__file__).resolve().parent / "golden"
 
 
 def _reference(name):
+    """(mean, std, spp, pixels or None) of a stored reference."""
     f = GOLD / name
     if not f.exists():
         pytest.skip(f"{name} not generated (tests/golden/make_fp64_references.py)")
     z = np.load(f)
-    return z["mean"].astype(np.float64), z["std"].astype(np.float64), int(z["spp"])
+    px = z["pixels"] if "pixels" in z.files else None
+    return (z["mean"].reshape(-1, 3).astype(np.float64), z["std"].reshape(-1, 3).astype(np.float64), int(z["spp"]),
+            px)
 
 
 def _frame(sc, spp):
@@ -51,10 +54,9 @@ def _check(img, mean, std, spp, ref_spp, frac=0.995, ratio=(0.9, 1.1)):
 
 def test_cornell_frame_matches_fp64_64k_reference():
     print("[ref] cornell 256^2 x 4096 spp", flush=True)
-    mean, std, ref_spp = _reference("cornell_c0_fp64_64k.npz")
+    mean, std, ref_spp, _ = _reference("cornell_c0_fp64_64k.npz")
     spp = 4096
     img = _frame(scenes.cornell(256, 256, max_depth=4), spp)
-    mean, std = mean.reshape(-1, 3), std.reshape(-1, 3)
     _check(img, mean, std, spp, ref_spp)
     # 8 x 8-pixel block means: no block biased
     blk = lambda a: a.reshape(32, 8, 32, 8, 3).mean((1, 3))  # noqa: E731
@@ -65,11 +67,7 @@ def test_cornell_frame_matches_fp64_64k_reference():
 
 def test_instanced_frame_matches_fp64_64k_reference_on_1024_pixels():
     print("[ref] instanced 1080p x 256 spp", flush=True)
-    z = np.load(GOLD / "instanced_c2_fp64_64k.npz") if (GOLD / "instanced_c2_fp64_64k.npz").exists() else None
-    if z is None:
-        pytest.skip("instanced_c2_fp64_64k.npz not generated (tests/golden/make_fp64_references.py)")
-    px = z["pixels"]
-    mean, std, ref_spp = z["mean"].astype(np.float64), z["std"].astype(np.float64), int(z["spp"])
+    mean, std, ref_spp, px = _reference("instanced_c2_fp64_64k.npz")
     spp = 256
     img = _frame(scenes.instanced(), spp)[px]
     # rare bright samples (sun lobe, lights seen after glossy bounces) carry
