@@ -396,12 +396,12 @@ def cornell_leg(stream, cpu=True):
     ctx = lfh.Context.from_scene(sc)
     ctx.set_stream(stream.cuda_stream)
     cam = sc.camera()
+    spp = 64
     ctx.update(cam)
-    _timed_passes(ctx, stream, 4)      # warm-up
+    _timed_passes(ctx, stream, spp)    # warm-up with the timed wavefront's size: its buffers exist before timing
     ctx.update(cam, reset=True)
     ctx.update(cam)
     r0 = ctx.ray_counts()
-    spp = 64
     ms = _timed_passes(ctx, stream, spp)
     r1 = ctx.ray_counts()
     samples = 256 * 256 * spp
@@ -428,11 +428,11 @@ def instanced_leg(stream, cpu=True):
     ctx = lfh.Context.from_scene(sc)
     ctx.set_stream(stream.cuda_stream)
     cam = sc.camera()
+    spp = 16
     ctx.update(cam)
-    _timed_passes(ctx, stream, 2)
+    _timed_passes(ctx, stream, spp)    # warm-up: buffers of the timed wavefronts exist before timing
     ctx.update(cam, reset=True)
     r0 = ctx.ray_counts()
-    spp = 16
     ms = _timed_passes(ctx, stream, spp)
     r1 = ctx.ray_counts()
     samples = 1920 * 1080 * spp

@@ -78,7 +78,7 @@ int lfh_create(const lfh_scene_desc* desc, const lfh_options* opt, lfh_ctx** out
         c->grid_blocks = lfh_trace_grid_blocks();
         size_t lanes = (size_t)c->grid_blocks * lfh_trace_threads_per_block();
         e = c->spill.alloc(lanes * lfh_trace_spill_stride() * sizeof(uint32_t));
-        if (e == hipSuccess) e = c->work.alloc(64);
+        if (e == hipSuccess) e = c->work.alloc(lfhd::kFetchWords * 8);
         if (e == hipSuccess) e = c->errflag.alloc(64);
         if (e == hipSuccess) e = hipMemsetAsync(c->errflag.p, 0, 64, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);

@@ -314,7 +314,7 @@ int lfh_set_frame_offset(lfh_ctx* c, int32_t offset) {
 int lfh_get_ray_counts(lfh_ctx* c, int64_t* extended, int64_t* shadow) {
     if (!c || !extended || !shadow) return rfail(c, LFH_E_INVALID, "lfh_get_ray_counts: NULL argument");
     unsigned long long t[2] = {0, 0};
-    RCHK(c, hipMemcpyAsync(t, c->pass.work.as<unsigned long long>() + 8, sizeof t, hipMemcpyDeviceToHost, c->stream),
+    RCHK(c, hipMemcpyAsync(t, c->pass.work.as<unsigned long long>() + lfhd::kWorkTotals, sizeof t, hipMemcpyDeviceToHost, c->stream),
          "lfh_get_ray_counts");
     RCHK(c, hipStreamSynchronize(c->stream), "lfh_get_ray_counts");
     *extended = (int64_t)t[0];

@@ -128,8 +128,8 @@ struct PassBufs {
         for (int i = 0; i < 10; ++i)
             if ((e = b[i]->alloc(f4 * m[i] * (size_t)n)) != hipSuccess) return e;
         if ((e = counters.alloc(64)) != hipSuccess) return e;
-        if ((e = work.alloc(128)) != hipSuccess) return e;
-        if ((e = hipMemset(work.p, 0, 128)) != hipSuccess) return e;
+        if ((e = work.alloc(lfhd::kWorkWords * 8)) != hipSuccess) return e;
+        if ((e = hipMemset(work.p, 0, lfhd::kWorkWords * 8)) != hipSuccess) return e;
         if ((e = job.alloc(4 * (size_t)n)) != hipSuccess) return e;
         cap = n;
         return hipSuccess;

@@ -58,6 +58,19 @@ LFH_HD uint32_t make_inst(uint32_t inst) { return kRefInstBit | inst; }
 #ifndef LFH_POP_CULL
 #define LFH_POP_CULL 0   // 1: keep box entry distances on the stack and drop stale entries at pop
 #endif
+// Work counters of the persistent traversal kernels: a queue's ray ids are
+// split into kFetchSlices contiguous slices, each with its own counter
+// kFetchStride 64-bit words (256 B) from the next.  The path tracer's work
+// buffer holds the extend queue's counters, then the shadow queue's, then
+// two running ray totals at kWorkTotals.
+#ifndef LFH_FETCH_SLICES
+#define LFH_FETCH_SLICES 8
+#endif
+constexpr int kFetchSlices = LFH_FETCH_SLICES;
+constexpr int kFetchStride = 32;
+constexpr int kFetchWords = kFetchSlices * kFetchStride;
+constexpr int kWorkTotals = 2 * kFetchWords;
+constexpr int kWorkWords = kWorkTotals + 8;
 constexpr int kLdsStack = LFH_LDS_STACK;  // LDS entries per lane (power of 2)
 constexpr int kTraceBlock = 256;        // threads per traversal block (4 wave64)
 constexpr uint32_t kSpillStride = 128;  // max stack entries per lane

@@ -70,6 +70,62 @@ struct LaneStack {
     }
 };
 
+// Work fetch of the persistent traversal kernels.  All fetches to one
+// counter serialise across the XCDs, so a queue's ids are split into
+// kFetchSlices contiguous slices with a counter each (kFetchStride words
+// apart).  A wave starts on slice blockIdx.x % kFetchSlices (blocks go to the
+// XCDs in turn, so each XCD starts on its own run of neighbouring rays),
+// reserves kFetchChunk ids at a time and hands them to its idle lanes over
+// several refills; when its slice runs out it moves to the next, and after
+// finding every slice exhausted it has no more work.  Counters only grow,
+// so a slice once exhausted stays so, and every id below n is handed out
+// exactly once.
+#ifndef LFH_FETCH_CHUNK
+#define LFH_FETCH_CHUNK 64
+#endif
+constexpr int kFetchChunk = LFH_FETCH_CHUNK;
+struct WaveFetch {
+    unsigned long long base = 0;   // next id of the wave's reservation (same in every lane)
+    int left = 0;                  // ids left in it
+    int slice = -1;                // current slice
+    int tried = 0;                 // slices found exhausted
+    // Ids for the lanes in `mask` (nonzero; called by every lane of the wave),
+    // in rank order within the mask; n (no work) for lanes left without one.
+    LFH_DEV int64_t next(unsigned long long* ctr, uint64_t mask, int lane, int64_t n, int first_slice) {
+        const int want = __popcll(mask);
+        const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+        if (left >= want) {
+            const int64_t id = (int64_t)base + rank;
+            base += (unsigned long long)want;
+            left -= want;
+            return id;
+        }
+        const int old = left;
+        int64_t id = rank < old ? (int64_t)base + rank : n;
+        left = 0;
+        if (slice < 0) slice = first_slice;
+        while (tried < kFetchSlices) {
+            const int64_t lo = n * slice / kFetchSlices, hi = n * (slice + 1) / kFetchSlices;
+            unsigned long long nb = 0;
+            if (lane == 0) nb = atomicAdd(ctr + (size_t)slice * kFetchStride, (unsigned long long)kFetchChunk);
+            nb = __shfl(nb, 0);
+            const int64_t start = lo + (int64_t)nb;
+            if (start >= hi) {
+                slice = slice + 1 == kFetchSlices ? 0 : slice + 1;
+                ++tried;
+                continue;
+            }
+            const int avail = hi - start < kFetchChunk ? (int)(hi - start) : kFetchChunk;
+            const int used = want - old < avail ? want - old : avail;
+            if (rank >= old && rank < old + used) id = start + (rank - old);
+            base = (unsigned long long)(start + used);
+            left = avail - used;
+            break;
+        }
+        return id;
+    }
+};
+
 struct TraceStats {
     int nint, ntri, ninst;
 };

@@ -13,7 +13,7 @@ struct TraceLaunch {
     uint8_t* occluded;        // 1 byte per ray
     int32_t* counts;          // 3 int32 per ray or NULL
     uint32_t* spill;          // grid_blocks * 256 * spill_stride
-    unsigned long long* work; // 1 counter
+    unsigned long long* work; // kFetchSlices counters, kFetchStride words apart
     int* err;                 // bit 0: stack overflow
     float cull;
     bool any;

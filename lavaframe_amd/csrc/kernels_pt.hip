@@ -29,6 +29,9 @@ LFH_DEV float* guide_rec(const GuideHook& g, int p, int depth) {
 }
 
 // ---- queue helpers ---------------------------------------------------------------
+#ifndef LFH_BLOCK_APPEND
+#define LFH_BLOCK_APPEND 1   // 1: queue appends take one atomic per block, 0: one per wave
+#endif
 // Wave-aggregated append: returns this lane's slot (valid only if `want`).
 LFH_DEV uint32_t wave_append(uint32_t* counter, bool want) {
     const uint64_t mask = __ballot(want);
@@ -41,24 +44,60 @@ LFH_DEV uint32_t wave_append(uint32_t* counter, bool want) {
     return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
 }
 
+// Block-aggregated appends: every thread of the block calls them at the same
+// point (the kernels using them have no early exit).  One atomic per block
+// instead of one per wave: all the device's appends to a queue hit the same
+// counter, whose atomics serialise across the XCDs.  `lds` holds 16 words.
+// Returns this thread's slot for `want` (and for `want_b`, appended after
+// the block's `want` entries of the same wave).
+LFH_DEV uint32_t block_append2(uint32_t* counter, bool want, bool want_b, uint32_t* slot_b, uint32_t* lds) {
+    const uint64_t ma = __ballot(want), mb = __ballot(want_b);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (int)(blockDim.x >> 6);
+    const uint32_t ca = (uint32_t)__popcll(ma);
+    if (lane == 0) lds[wave] = ca + (uint32_t)__popcll(mb);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (int w = 0; w < nw; ++w) tot += lds[w];
+        uint32_t base = tot ? atomicAdd(counter, tot) : 0u;
+        for (int w = 0; w < nw; ++w) {
+            const uint32_t c = lds[w];
+            lds[w] = base;
+            base += c;
+        }
+    }
+    __syncthreads();
+    const uint64_t below = (1ull << lane) - 1ull;
+    const uint32_t wb = lds[wave];
+    *slot_b = wb + ca + (uint32_t)__popcll(mb & below);
+    const uint32_t sa = wb + (uint32_t)__popcll(ma & below);
+    __syncthreads();   // lds is free for the next append
+    return sa;
+}
+LFH_DEV uint32_t block_append(uint32_t* counter, bool want, uint32_t* lds) {
+    uint32_t unused;
+    return block_append2(counter, want, false, &unused, lds);
+}
+
 LFH_DEV float4 f4(V3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
 
 __global__ void k_pt_reset(uint32_t* counters, unsigned long long* work) {
     int i = threadIdx.x;
     if (i < 8) counters[i] = 0u;
-    if (i < 8) work[i] = 0ull;
+    if (i < 2 * kFetchSlices) work[i * kFetchStride] = 0ull;
 }
 
-// work[0..7]: per-launch fetch counters; work[8], work[9]: running totals of
-// extended and shadow rays (never reset by a pass).
+// work: the extend and shadow queues' fetch counters (zeroed after every
+// bounce), then at kWorkTotals the running totals of extended and shadow
+// rays (never reset by a pass).
 __global__ void k_pt_advance(uint32_t* consumed, uint32_t* shadow_count, unsigned long long* work) {
     if (threadIdx.x == 0) {
-        work[8] += *consumed;
-        work[9] += *shadow_count;
+        work[kWorkTotals] += *consumed;
+        work[kWorkTotals + 1] += *shadow_count;
         *consumed = 0u;
         *shadow_count = 0u;
     }
-    if (threadIdx.x < 8) work[threadIdx.x] = 0ull;
+    if (threadIdx.x < 2 * kFetchSlices) work[threadIdx.x * kFetchStride] = 0ull;
 }
 
 // generate: one thread per tile pixel (MULTI: per pixel of every job)
@@ -107,7 +146,12 @@ __global__ __launch_bounds__(256) void k_generate(CamParams cam1, const JobDev* 
         pb.rad[p] = make_float4(0.f, 0.f, 0.f, 0.f);
         pb.nee[p * 3 + 2] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
+#if LFH_BLOCK_APPEND
+    __shared__ uint32_t app[16];
+    uint32_t slot = block_append(count, valid, app);
+#else
     uint32_t slot = wave_append(count, valid);
+#endif
     if (valid) {
         rayq[(size_t)slot * 2] = f4(o, __int_as_float(p));
         rayq[(size_t)slot * 2 + 1] = f4(d, 0.f);
@@ -128,7 +172,12 @@ __global__ __launch_bounds__(256) void k_generate_rays(const float* __restrict__
         pb.misc[p] = make_float4(0.f, __int_as_float(0), 0.f, 0.f);
         pb.nee[p * 3 + 2] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
+#if LFH_BLOCK_APPEND
+    __shared__ uint32_t app[16];
+    uint32_t slot = block_append(count, valid, app);
+#else
     uint32_t slot = wave_append(count, valid);
+#endif
     if (valid) {
         const float* r = rays6 + (size_t)p * 6;
         rayq[(size_t)slot * 2] = make_float4(r[0], r[1], r[2], __int_as_float(p));
@@ -207,16 +256,14 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_queue(ShadeScene sc, cons
     int light = -1;
     float lpdf = 0.f;
     TraceScene ts = sc.tr;
+    WaveFetch fetch;
     for (;;) {
         const bool need = !active;
         const uint64_t mask = __ballot(need);
         if (mask) {
-            const int leader = __ffsll((long long)mask) - 1;
-            unsigned long long base = 0;
-            if (lane == leader) base = atomicAdd(work, (unsigned long long)__popcll(mask));
-            base = __shfl(base, leader);
+            const int64_t id = fetch.next(work, mask, lane, n, (int)(blockIdx.x % kFetchSlices));
             if (need) {
-                rid = (int64_t)base + __popcll(mask & ((1ull << lane) - 1ull));
+                rid = id;
                 if (rid < n) {
                     float4 a = q[rid * 2], b = q[rid * 2 + 1];
                     V3 o = v3(a.x, a.y, a.z), d = v3(b.x, b.y, b.z);
@@ -286,7 +333,10 @@ LFH_DEV void combine_nee(PathBufs& pb, int p, V3& rad) {
 }
 
 template <bool MULTI>
-__global__ __launch_bounds__(256) void k_shade(ShadeScene sc1, const JobDev* __restrict__ jobs, int depth,
+#ifndef LFH_SHADE_WAVES
+#define LFH_SHADE_WAVES 1   // k_shade register budget in waves per SIMD; 1 (165 VGPRs, 3 waves): C0 400 / C2 249 Msamples/s, 4 (156 B/lane spilled): 414 / 249, 5: 409 / 248
+#endif
+__global__ __launch_bounds__(256, LFH_SHADE_WAVES) void k_shade(ShadeScene sc1, const JobDev* __restrict__ jobs, int depth,
                                                const float4* __restrict__ rayq,
                                                const float4* __restrict__ hitq, const uint32_t* __restrict__ count,
                                                float4* __restrict__ nextq, uint32_t* __restrict__ next_count,
@@ -499,17 +549,24 @@ __global__ __launch_bounds__(256) void k_shade(ShadeScene sc1, const JobDev* __r
         pb.rad[p] = f4(rad, 0.f);
         pb.misc[p] = make_float4(prevPdf, __int_as_float(lastMat), texc.x, texc.y);
     }
+#if LFH_BLOCK_APPEND
+    __shared__ uint32_t app[16];
+    uint32_t s = block_append(next_count, cont, app);
+    uint32_t sl;
+    uint32_t se = block_append2(shadow_count, shEnv, shLight, &sl, app);
+#else
     uint32_t s = wave_append(next_count, cont);
+    uint32_t se = wave_append(shadow_count, shEnv);
+    uint32_t sl = wave_append(shadow_count, shLight);
+#endif
     if (cont) {
         nextq[(size_t)s * 2] = f4(nextO, __int_as_float(p));
         nextq[(size_t)s * 2 + 1] = f4(nextD, 0.f);
     }
-    uint32_t se = wave_append(shadow_count, shEnv);
     if (shEnv) {
         shadowq[(size_t)se * 2] = f4(shO, __uint_as_float(((uint32_t)p << 1) | 0u));
         shadowq[(size_t)se * 2 + 1] = f4(shEnvD, kInfinity - kEps);
     }
-    uint32_t sl = wave_append(shadow_count, shLight);
     if (shLight) {
         shadowq[(size_t)sl * 2] = f4(shO, __uint_as_float(((uint32_t)p << 1) | 1u));
         shadowq[(size_t)sl * 2 + 1] = f4(shLightD, shLightMax);
@@ -726,7 +783,7 @@ hipError_t pt_bounce(const ShadeScene& sc, int depth, int maxq, float4* rayq, ui
     hipLaunchKernelGGL(k_shade<false>, dim3(nblk(maxq)), dim3(256), 0, s, sc, nj, depth, rayq, hitq, count, nextq,
                        next_count, shadowq, shadow_count, pb, gh);
     hipLaunchKernelGGL((k_trace_queue<true, false>), dim3(grid_blocks), dim3(kTraceBlock), 0, s, sc, nj, shadowq,
-                       shadow_count, hitq, pb, spill, work + 1, cull, err);
+                       shadow_count, hitq, pb, spill, work + kFetchWords, cull, err);
     hipLaunchKernelGGL(k_pt_advance, dim3(1), dim3(64), 0, s, count, shadow_count, work);
     return hipGetLastError();
 }
@@ -742,7 +799,7 @@ hipError_t pt_bounce_multi(const JobDev* jobs, int depth, int maxq, float4* rayq
     hipLaunchKernelGGL(k_shade<true>, dim3(nblk(maxq)), dim3(256), 0, s, dummy, jobs, depth, rayq, hitq, count, nextq,
                        next_count, shadowq, shadow_count, pb, gh);
     hipLaunchKernelGGL((k_trace_queue<true, true>), dim3(grid_blocks), dim3(kTraceBlock), 0, s, dummy, jobs, shadowq,
-                       shadow_count, hitq, pb, spill, work + 1, cull, err);
+                       shadow_count, hitq, pb, spill, work + kFetchWords, cull, err);
     hipLaunchKernelGGL(k_pt_advance, dim3(1), dim3(64), 0, s, count, shadow_count, work);
     return hipGetLastError();
 }

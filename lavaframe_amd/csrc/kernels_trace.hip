@@ -2,17 +2,16 @@
 //
 // One launch fills the chip with exactly as many 256-thread blocks as can be
 // resident.  Lanes fetch rays dynamically: every idle lane of a wave asks for
-// work in the same step, one atomic per wave reserves a contiguous range, and
-// each lane takes base + (its rank among the idle lanes), so refills stay
-// coalesced while busy lanes keep traversing.
+// work in the same step and takes the next ids of a contiguous chunk its wave
+// reserved with one atomic (WaveFetch), in rank order among the idle lanes, so
+// refills stay coalesced while busy lanes keep traversing.
 #include "dev_trace.h"
 #include "kernels.h"
 
 namespace lfhd {
 
-__global__ void k_zero_u64(unsigned long long* p, int n) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0ull;
+__global__ void k_zero_fetch(unsigned long long* work) {
+    if (threadIdx.x < kFetchSlices) work[threadIdx.x * kFetchStride] = 0ull;
 }
 
 template <bool ANY, bool STATS>
@@ -36,18 +35,15 @@ __global__ __launch_bounds__(kTraceBlock, LFH_TRACE_WAVES) void k_trace(TraceSce
     TraceStats stats{0, 0, 0};
     bool active = false;
     int64_t rid = 0;
+    WaveFetch fetch;
 
     for (;;) {
         const bool need = !active;
         const uint64_t mask = __ballot(need);
         if (mask) {
-            const int leader = __ffsll((long long)mask) - 1;
-            unsigned long long base = 0;
-            if (lane == leader) base = atomicAdd(work, (unsigned long long)__popcll(mask));
-            base = __shfl(base, leader);
+            const int64_t id = fetch.next(work, mask, lane, n, (int)(blockIdx.x % kFetchSlices));
             if (need) {
-                const int rank = __popcll(mask & ((1ull << lane) - 1ull));
-                rid = (int64_t)base + rank;
+                rid = id;
                 if (rid < n) {
                     float4 a = rays[rid * 2], b = rays[rid * 2 + 1];
                     trace_init(s, st, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, b.w, sc.root);
@@ -113,7 +109,7 @@ uint32_t lfh_trace_spill_stride() { return kSpillStride; }
 
 hipError_t lfh_launch_trace(const TraceLaunch& L, hipStream_t stream) {
     const int blocks = L.grid_blocks;
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, stream, L.work, 1);
+    hipLaunchKernelGGL(k_zero_fetch, dim3(1), dim3(64), 0, stream, L.work);
     if (L.any) {
         hipLaunchKernelGGL((k_trace<true, false>), dim3(blocks), dim3(kTraceBlock), 0, stream, L.scene, L.rays, L.n,
                            L.hits, L.occluded, L.counts, L.spill, L.work, L.cull, L.err);
