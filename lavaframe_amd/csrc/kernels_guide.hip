@@ -47,6 +47,11 @@ LFH_DEV void encode_row(uint16_t* row, const GuideDev& g, V3 p, V3 n, V3 wo) {
 // 16-row tiles), in place: hidden layers overwrite X, the 40 outputs go to O
 // (FP32, kOutPitch).  Each 16-row tile reads only its own rows, so the
 // in-place update is safe within the wave.
+// O may overlay X (one buffer per wave): the output layer takes the row tiles
+// last to first, and O's rows of tile mt (byte offsets from mt * 16 *
+// kOutPitch * 4 on) then lie at or above the activations of the tiles not yet
+// read (below mt * 16 * kPitch * 2), while tile mt's own activations are in
+// registers before its outputs are stored.
 LFH_DEV void mlp_forward_rows(const GuideDev& g, uint16_t* X, float* O, int ntiles) {
     const int lane = threadIdx.x & 63;
     const int lr = lane & 15, lk = (lane >> 4) * 8, lq = (lane >> 4) * 4;
@@ -66,7 +71,8 @@ LFH_DEV void mlp_forward_rows(const GuideDev& g, uint16_t* X, float* O, int ntil
 #pragma unroll
             for (int ks = 0; ks < 2; ++ks) bw[nt][ks] = ld_frag(W + (nt * 16 + lr) * 64 + ks * 32 + lk);
         }
-        for (int mt = 0; mt < ntiles; ++mt) {
+        for (int it = 0; it < ntiles; ++it) {
+            const int mt = layer < kGuideLayers - 1 ? it : ntiles - 1 - it;
             f32x4 acc[4];
 #pragma unroll
             for (int nt = 0; nt < 4; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -77,6 +83,9 @@ LFH_DEV void mlp_forward_rows(const GuideDev& g, uint16_t* X, float* O, int ntil
                 for (int nt = 0; nt < 4; ++nt)
                     if (nt < nN) acc[nt] = mfma(a, bw[nt][ks], acc[nt]);
             }
+            // the tile's activation reads stay before its stores even where
+            // O (float) overlays X (uint16_t)
+            __asm__ volatile("" ::: "memory");
 #pragma unroll
             for (int nt = 0; nt < 4; ++nt) {
                 if (nt >= nN) continue;
@@ -148,8 +157,9 @@ template <bool QUEUE>
 __global__ __launch_bounds__(kInferBlock) void k_guide_infer(GuideDev g, const float* __restrict__ in9, int n_host,
                                                              InferQueue q, float* __restrict__ raw40,
                                                              float4* __restrict__ lobes) {
-    __shared__ __attribute__((aligned(16))) uint16_t sX[kInferWaves][64 * kPitch];
-    __shared__ float sO[kInferWaves][64 * kOutPitch];
+    // one buffer per wave: BF16 activations, then FP32 outputs over them
+    constexpr int kWaveBytes = 64 * kOutPitch * 4 > 64 * kPitch * 2 ? 64 * kOutPitch * 4 : 64 * kPitch * 2;
+    __shared__ __attribute__((aligned(16))) uint8_t sU[kInferWaves][kWaveBytes];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t n = QUEUE ? *q.count : (uint32_t)n_host;
     if ((uint32_t)blockIdx.x * kInferBlock >= n) return;  // block-uniform
@@ -169,8 +179,8 @@ __global__ __launch_bounds__(kInferBlock) void k_guide_infer(GuideDev g, const f
             wo = v3(s[6], s[7], s[8]);
         }
     }
-    uint16_t* X = sX[wave];
-    float* O = sO[wave];
+    uint16_t* X = reinterpret_cast<uint16_t*>(sU[wave]);
+    float* O = reinterpret_cast<float*>(sU[wave]);
     encode_row(X + lane * kPitch, g, pos, nrm, wo);
     __syncthreads();
     mlp_forward_rows(g, X, O, 4);
