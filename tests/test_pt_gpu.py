@@ -279,6 +279,25 @@ def test_kahan_accumulation():
     assert np.abs(out[0] - exact).max() > np.abs(out[1] - exact).max()
 
 
+def test_path_batches_of_any_size_equal_one_large_batch():
+    """Queue appends are aggregated per 256-thread block and traversal work is
+    fetched from 8 slices of each queue: paths traced in batches of 1, 5, 65
+    and the rest are bitwise those of one batch (no path lost, duplicated or
+    mixed with another)."""
+    print("[pt] batch sizes", flush=True)
+    sc = scenes.cornell(64, 64)
+    ctx = ctx_of(sc)
+    rng = np.random.default_rng(31)
+    n = 700
+    rays = _camera_rays(sc, n, rng)
+    seeds = rng.integers(0, 2**32, (n, 4), dtype=np.uint64).astype(np.uint32)
+    full = ctx.path_trace_rays(rays, seeds)
+    assert full.sum() > 0
+    for a, b in ((0, 1), (1, 6), (6, 71), (71, n)):
+        part = ctx.path_trace_rays(rays[a:b], seeds[a:b])
+        assert np.array_equal(part.view(np.uint32), full[a:b].view(np.uint32)), (a, b)
+
+
 def test_batched_passes_equal_sequential_passes():
     """lfh_render_passes over a one-tile frame renders many passes in one
     wavefront; the accumulation (sum and compensation), sample counter and

@@ -101,6 +101,21 @@ def test_empty_batch_and_device_entry():
     _compare(gh, ctx.trace_closest(rays))
 
 
+def test_small_and_odd_batches_trace_every_ray():
+    """Ray ids are fetched from 8 contiguous slices of the batch, each with its
+    own counter: batches smaller than the slice count (slices of 0 or 1 ray)
+    and sizes just around a wave's width must still trace every ray once."""
+    print("[trace] small batches", flush=True)
+    sc = scenes.soup(2000, seed=21)
+    ctx, arr = _ctx(sc)
+    orc = native.OracleScene(arr)
+    rays = aimed_rays(sc, 1100, seed=22)
+    for n in (1, 2, 5, 7, 8, 9, 63, 64, 65, 127, 129, 1031):
+        r = np.ascontiguousarray(rays[:n])
+        _compare(ctx.trace_closest(r), orc.trace_closest(r, tie_mode=1))
+        assert np.array_equal(ctx.trace_any(r), orc.trace_any(r)), n
+
+
 def _leaving_rays(ctx, n, seed):
     """Continuation-like rays: from the hit points of random rays, offset
     1e-3 along a new random direction (pathtrace.glsl:291), many of them
