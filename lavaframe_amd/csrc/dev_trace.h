@@ -167,7 +167,101 @@ LFH_DEV void trace_init(TraceState& s, LaneStack& st, V3 O, V3 D, float tmin, fl
     st.reset();
 }
 
-// Runs up to `budget` node visits; returns true when the traversal finished.
+// Triangle tests of BLAS leaf `ref` (in the current instance's space).
+// Returns true when an any-hit query found its hit.
+template <bool ANY, bool STATS>
+LFH_DEV bool leaf_test(const TraceScene& sc, TraceState& s, uint32_t ref, TraceStats* stats) {
+    const uint32_t first = leaf_first(ref), count = leaf_count(ref);
+    for (uint32_t k = 0; k < count; ++k) {
+        const float4* tp = sc.tris + (size_t)(first + k) * 3;
+        float4 a = tp[0], b = tp[1], c = tp[2];
+        float t, u, v;
+        if (STATS) stats->ntri++;
+        bool h = tri_test(s.o, s.d, xyz(a), xyz(b), xyz(c), &t, &u, &v);
+        if (ANY) {
+            if (h && t >= s.tmin && t < s.tbest) {
+                s.occluded = true;
+                return true;
+            }
+        } else if (h && t >= s.tmin) {
+            int prim = (int)as_u(a.w);
+            bool better = t < s.tbest ||
+                          (t == s.tbest && (s.curInst < s.bestInst ||
+                                            (s.curInst == s.bestInst && prim < s.bestPrim)));
+            if (better) {
+                s.tbest = t;
+                s.bestPrim = prim;
+                s.bestSlot = (int)as_u(b.w);
+                s.bestInst = s.curInst;
+                s.bestMat = s.curMat;
+                s.bu = u;
+                s.bv = v;
+            }
+        }
+    }
+    return false;
+}
+
+// TLAS leaf: move the ray into instance space and continue at the BLAS root.
+template <bool STATS>
+LFH_DEV void enter_instance(const TraceScene& sc, TraceState& s, LaneStack& st, uint32_t cur, TraceStats* stats) {
+    const uint32_t inst = cur & 0x3FFFFFFFu;
+    const float4* ip = sc.insts + (size_t)inst * 4;
+    float4 r0 = ip[0], r1 = ip[1], r2 = ip[2], meta = ip[3];
+    if (STATS) stats->ninst++;
+    s.o = xform_inv(r0, r1, r2, s.O, 1.0f);
+    s.d = xform_inv(r0, r1, r2, s.D, 0.0f);
+    s.inv = rcp3(s.d);
+    st.push(kRefSentinel, -kInfinity);
+    s.inBlas = true;
+    s.curInst = (int)inst;
+    s.curMat = (int)as_u(meta.y);
+    s.cur = as_u(meta.x);
+}
+
+// BVH4 node: slab-test the four child boxes, descend into the nearest hit
+// child, push the others farthest first.
+template <bool STATS>
+LFH_DEV void visit_node(const TraceScene& sc, TraceState& s, LaneStack& st, uint32_t cur, float cull,
+                        TraceStats* stats) {
+    const float4* np = sc.nodes + (size_t)cur * 8;
+    const float4 mnx = np[0], mny = np[1], mnz = np[2], mxx = np[3], mxy = np[4], mxz = np[5];
+    const float4 rf = np[6];
+    if (STATS) stats->nint++;
+    const float lim = s.tbest * cull;
+    float h[4], e[4];
+    uint32_t r[4] = {as_u(rf.x), as_u(rf.y), as_u(rf.z), as_u(rf.w)};
+    h[0] = slab(v3(mnx.x, mny.x, mnz.x), v3(mxx.x, mxy.x, mxz.x), s.o, s.inv, &e[0]);
+    h[1] = slab(v3(mnx.y, mny.y, mnz.y), v3(mxx.y, mxy.y, mxz.y), s.o, s.inv, &e[1]);
+    h[2] = slab(v3(mnx.z, mny.z, mnz.z), v3(mxx.z, mxy.z, mxz.z), s.o, s.inv, &e[2]);
+    h[3] = slab(v3(mnx.w, mny.w, mnz.w), v3(mxx.w, mxy.w, mxz.w), s.o, s.inv, &e[3]);
+    int nv = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const bool ok = r[k] != kRefSentinel && h[k] > 0.f && e[k] <= lim;
+        nv += ok ? 1 : 0;
+        h[k] = ok ? fminf(h[k], 3.0e38f) : __builtin_inff();
+    }
+    // sort by hit distance (absent / culled children last)
+#define LFH_CSWAP(a, b)                                              \
+    if (h[b] < h[a]) {                                               \
+        float th = h[a]; h[a] = h[b]; h[b] = th;                     \
+        float te = e[a]; e[a] = e[b]; e[b] = te;                     \
+        uint32_t tr = r[a]; r[a] = r[b]; r[b] = tr;                  \
+    }
+    LFH_CSWAP(0, 1) LFH_CSWAP(2, 3) LFH_CSWAP(0, 2) LFH_CSWAP(1, 3) LFH_CSWAP(1, 2)
+#undef LFH_CSWAP
+    if (nv == 0) {
+        s.cur = st.pop_live(lim);
+    } else {
+        if (nv > 3) st.push(r[3], e[3]);
+        if (nv > 2) st.push(r[2], e[2]);
+        if (nv > 1) st.push(r[1], e[1]);
+        s.cur = r[0];
+    }
+}
+
+// Runs up to `budget` loop steps; returns true when the traversal finished.
 template <bool ANY, bool STATS>
 LFH_DEV bool trace_run(const TraceScene& sc, TraceState& s, LaneStack& st, float cull, int budget,
                        TraceStats* stats) {
@@ -185,86 +279,12 @@ LFH_DEV bool trace_run(const TraceScene& sc, TraceState& s, LaneStack& st, float
             return true;
         }
         if (ref_is_leaf(cur)) {
-            const uint32_t first = leaf_first(cur), count = leaf_count(cur);
-            for (uint32_t k = 0; k < count; ++k) {
-                const float4* tp = sc.tris + (size_t)(first + k) * 3;
-                float4 a = tp[0], b = tp[1], c = tp[2];
-                float t, u, v;
-                if (STATS) stats->ntri++;
-                bool h = tri_test(s.o, s.d, xyz(a), xyz(b), xyz(c), &t, &u, &v);
-                if (ANY) {
-                    if (h && t >= s.tmin && t < s.tbest) {
-                        s.occluded = true;
-                        return true;
-                    }
-                } else if (h && t >= s.tmin) {
-                    int prim = (int)as_u(a.w);
-                    bool better = t < s.tbest ||
-                                  (t == s.tbest && (s.curInst < s.bestInst ||
-                                                    (s.curInst == s.bestInst && prim < s.bestPrim)));
-                    if (better) {
-                        s.tbest = t;
-                        s.bestPrim = prim;
-                        s.bestSlot = (int)as_u(b.w);
-                        s.bestInst = s.curInst;
-                        s.bestMat = s.curMat;
-                        s.bu = u;
-                        s.bv = v;
-                    }
-                }
-            }
+            if (leaf_test<ANY, STATS>(sc, s, cur, stats)) return true;
             s.cur = st.pop_live(s.tbest * cull);
         } else if (ref_is_inst(cur)) {
-            const uint32_t inst = cur & 0x3FFFFFFFu;
-            const float4* ip = sc.insts + (size_t)inst * 4;
-            float4 r0 = ip[0], r1 = ip[1], r2 = ip[2], meta = ip[3];
-            if (STATS) stats->ninst++;
-            s.o = xform_inv(r0, r1, r2, s.O, 1.0f);
-            s.d = xform_inv(r0, r1, r2, s.D, 0.0f);
-            s.inv = rcp3(s.d);
-            st.push(kRefSentinel, -kInfinity);
-            s.inBlas = true;
-            s.curInst = (int)inst;
-            s.curMat = (int)as_u(meta.y);
-            s.cur = as_u(meta.x);
+            enter_instance<STATS>(sc, s, st, cur, stats);
         } else {
-            // BVH4 node: slab-test the four child boxes, descend into the
-            // nearest hit child, push the others farthest first
-            const float4* np = sc.nodes + (size_t)cur * 8;
-            const float4 mnx = np[0], mny = np[1], mnz = np[2], mxx = np[3], mxy = np[4], mxz = np[5];
-            const float4 rf = np[6];
-            if (STATS) stats->nint++;
-            const float lim = s.tbest * cull;
-            float h[4], e[4];
-            uint32_t r[4] = {as_u(rf.x), as_u(rf.y), as_u(rf.z), as_u(rf.w)};
-            h[0] = slab(v3(mnx.x, mny.x, mnz.x), v3(mxx.x, mxy.x, mxz.x), s.o, s.inv, &e[0]);
-            h[1] = slab(v3(mnx.y, mny.y, mnz.y), v3(mxx.y, mxy.y, mxz.y), s.o, s.inv, &e[1]);
-            h[2] = slab(v3(mnx.z, mny.z, mnz.z), v3(mxx.z, mxy.z, mxz.z), s.o, s.inv, &e[2]);
-            h[3] = slab(v3(mnx.w, mny.w, mnz.w), v3(mxx.w, mxy.w, mxz.w), s.o, s.inv, &e[3]);
-            int nv = 0;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const bool ok = r[k] != kRefSentinel && h[k] > 0.f && e[k] <= lim;
-                nv += ok ? 1 : 0;
-                h[k] = ok ? fminf(h[k], 3.0e38f) : __builtin_inff();
-            }
-            // sort by hit distance (absent / culled children last)
-#define LFH_CSWAP(a, b)                                              \
-    if (h[b] < h[a]) {                                               \
-        float th = h[a]; h[a] = h[b]; h[b] = th;                     \
-        float te = e[a]; e[a] = e[b]; e[b] = te;                     \
-        uint32_t tr = r[a]; r[a] = r[b]; r[b] = tr;                  \
-    }
-            LFH_CSWAP(0, 1) LFH_CSWAP(2, 3) LFH_CSWAP(0, 2) LFH_CSWAP(1, 3) LFH_CSWAP(1, 2)
-#undef LFH_CSWAP
-            if (nv == 0) {
-                s.cur = st.pop_live(lim);
-            } else {
-                if (nv > 3) st.push(r[3], e[3]);
-                if (nv > 2) st.push(r[2], e[2]);
-                if (nv > 1) st.push(r[1], e[1]);
-                s.cur = r[0];
-            }
+            visit_node<STATS>(sc, s, st, cur, cull, stats);
         }
         if (st.overflow) return true;
     }
