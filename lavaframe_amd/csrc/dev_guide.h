@@ -131,8 +131,10 @@ LFH_DEV void guide_lobes(const float* o, float4* out) {
     }
 }
 
-// True when the processed lobes of a query are usable (see guide_lobes).
-LFH_DEV bool guide_valid(const float4* __restrict__ lob) { return lob[8].x >= 0.0f; }
+// The processed lobes of one query are lob[k * st], k = 0..9 (st = 1: ten
+// consecutive float4; larger st: one query of a structure-of-arrays buffer).
+// True when they are usable (see guide_lobes).
+LFH_DEV bool guide_valid(const float4* __restrict__ lob, size_t st = 1) { return lob[8 * st].x >= 0.0f; }
 
 // Stores master parameter j (flat layout) into the BF16 operand copies:
 // wf[layer][out][in], wb[layer][in][out], or the FP32 bias.
@@ -152,13 +154,13 @@ LFH_DEV void guide_store_param(uint16_t* wf, uint16_t* wb, float* bias, int j, f
 
 LFH_DEV float vmf_norm(float kappa) { return kappa / (kTwoPi * -expm1f(-2.0f * kappa)); }
 
-LFH_DEV float guide_pdf(const float4* __restrict__ lob, V3 d) {
-    const float4 pa = lob[8], pb = lob[9];
+LFH_DEV float guide_pdf(const float4* __restrict__ lob, V3 d, size_t st = 1) {
+    const float4 pa = lob[8 * st], pb = lob[9 * st];
     const float pis[8] = {pa.x, pa.y, pa.z, pa.w, pb.x, pb.y, pb.z, pb.w};
     float p = 0.0f;
 #pragma unroll
     for (int k = 0; k < kGuideLobes; ++k) {
-        const float4 l = lob[k];
+        const float4 l = lob[k * st];
         const float c = l.x * d.x + l.y * d.y + l.z * d.z;
         p += pis[k] * vmf_norm(l.w) * expf(l.w * (c - 1.0f));
     }
@@ -179,8 +181,8 @@ LFH_DEV void guide_frame(V3 n, V3& t, V3& b) {
 // inverse CDF of cos(theta) (Jakob 2012, "Numerically stable sampling of the
 // von Mises Fisher distribution on S2"): w = 1 + log(u1 + (1-u1) e^{-2 kappa}) / kappa,
 // phi = 2 pi u2.
-LFH_DEV V3 guide_sample(const float4* __restrict__ lob, float u0, float u1, float u2) {
-    const float4 pa = lob[8], pb = lob[9];
+LFH_DEV V3 guide_sample(const float4* __restrict__ lob, float u0, float u1, float u2, size_t st = 1) {
+    const float4 pa = lob[8 * st], pb = lob[9 * st];
     const float pis[8] = {pa.x, pa.y, pa.z, pa.w, pb.x, pb.y, pb.z, pb.w};
     int k = kGuideLobes - 1;
     float cum = 0.0f;
@@ -192,7 +194,7 @@ LFH_DEV V3 guide_sample(const float4* __restrict__ lob, float u0, float u1, floa
     float4 l = lob[0];
 #pragma unroll
     for (int j = 1; j < kGuideLobes; ++j)
-        if (j == k) l = lob[j];
+        if (j == k) l = lob[j * st];
     const float kappa = l.w;
     float w = 1.0f + logf(u1 + (1.0f - u1) * expf(-2.0f * kappa)) / kappa;
     w = fminf(fmaxf(w, -1.0f), 1.0f);

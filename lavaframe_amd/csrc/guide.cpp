@@ -253,6 +253,7 @@ int guide_pass_begin(lfh_ctx* c, int batch, GuideHook* gh, lfhd::GuideDev* gd) {
     }
     gh->enabled = 1;
     gh->lobes = b.lobes.as<const float4>();
+    gh->lobe_stride = c->pass.cap;
     gh->max_depth = c->opt.max_depth;
     *gd = g->dev();
     if (training_passes(c, batch) > 0) {

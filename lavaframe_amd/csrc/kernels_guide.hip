@@ -197,9 +197,11 @@ __global__ __launch_bounds__(kInferBlock) void k_guide_infer(GuideDev g, const f
             for (int k = 0; k < kGuideOut; ++k) ov[k] = o[k];
             float4 lb[kGuideLobeF4];
             guide_lobes(ov, lb);
-            float4* dst = lobes + (size_t)i * kGuideLobeF4;
+            // the hit queue's lobes in structure-of-arrays order, host queries' packed
+            const size_t st = QUEUE ? (size_t)q.gh.lobe_stride : 1;
+            float4* dst = lobes + (QUEUE ? (size_t)i : (size_t)i * kGuideLobeF4);
 #pragma unroll
-            for (int k = 0; k < kGuideLobeF4; ++k) dst[k] = lb[k];
+            for (int k = 0; k < kGuideLobeF4; ++k) dst[k * st] = lb[k];
         }
     }
 }

@@ -433,9 +433,10 @@ __global__ __launch_bounds__(256, LFH_SHADE_WAVES) void k_shade(ShadeScene sc1, 
             lastMat = matId;
             thr = thr * v3(expf(-absorb.x * t), expf(-absorb.y * t), expf(-absorb.z * t));
             uint4 rng = pb.rng[p];
-            const float4* lob = gh.lobes + (size_t)i * kGuideLobeF4;
+            const float4* lob = gh.lobes + i;
+            const size_t lst = (size_t)gh.lobe_stride;
             // the strategy choice depends only on the lobes, never on a sample
-            const bool guided = gh.enabled != 0 && guide_valid(lob);
+            const bool guided = gh.enabled != 0 && guide_valid(lob, lst);
             // next-event estimation (pathtrace.glsl:126-204)
             const V3 V = -rd;
             // the reference offsets along the unflipped normal (pathtrace.glsl:130), so
@@ -451,7 +452,7 @@ __global__ __launch_bounds__(256, LFH_SHADE_WAVES) void k_shade(ShadeScene sc1, 
                 float bp;
                 V3 f = DisneyEval(m, fr.eta, V, fr.N, ldir, bp);
                 if (bp > 0.0f) {
-                    const float qp = guided ? (1.0f - kGuideFrac) * bp + kGuideFrac * guide_pdf(lob, ldir) : bp;
+                    const float qp = guided ? (1.0f - kGuideFrac) * bp + kGuideFrac * guide_pdf(lob, ldir, lst) : bp;
                     float mw = powerHeuristic(lightPdf, qp);
                     if (mw > 0.0f) {
                         V3 c = mw * f * fabsf(dot(ldir, fr.N)) * color / lightPdf;
@@ -473,7 +474,7 @@ __global__ __launch_bounds__(256, LFH_SHADE_WAVES) void k_shade(ShadeScene sc1, 
                     float weight = 1.0f;
                     if (area > 0.0f)
                         weight = powerHeuristic(
-                            lr.pdf, guided ? (1.0f - kGuideFrac) * bp + kGuideFrac * guide_pdf(lob, lr.direction) : bp);
+                            lr.pdf, guided ? (1.0f - kGuideFrac) * bp + kGuideFrac * guide_pdf(lob, lr.direction, lst) : bp);
                     if (bp > 0.0f) {
                         V3 c = weight * f * fabsf(dot(fr.N, lr.direction)) * lr.emission / lr.pdf;
                         l4 = f4(c, 1.0f);
@@ -498,7 +499,7 @@ __global__ __launch_bounds__(256, LFH_SHADE_WAVES) void k_shade(ShadeScene sc1, 
                 bool ok = true;
                 if (rnd(rng) < kGuideFrac) {
                     const float u0 = rnd(rng), u1 = rnd(rng), u2 = rnd(rng);
-                    L = guide_sample(lob, u0, u1, u2);
+                    L = guide_sample(lob, u0, u1, u2, lst);
                 } else {
                     float pl;
                     DisneySample(m, fr, V, L, pl, rng);
@@ -506,7 +507,7 @@ __global__ __launch_bounds__(256, LFH_SHADE_WAVES) void k_shade(ShadeScene sc1, 
                 }
                 float bp;
                 f = DisneyEval(m, fr.eta, V, fr.N, L, bp);
-                pdf = ok ? (1.0f - kGuideFrac) * bp + kGuideFrac * guide_pdf(lob, L) : 0.0f;
+                pdf = ok ? (1.0f - kGuideFrac) * bp + kGuideFrac * guide_pdf(lob, L, lst) : 0.0f;
                 misPdf = pdf;
             } else {
                 f = DisneySample(m, fr, V, L, pdf, rng);

@@ -32,7 +32,10 @@ struct GuideHook {
     int stride = 1;
     int phase = 0;
     float4* rec = nullptr;
+    // processed lobes of hit-queue entry i: float4 k at lobes[k * lobe_stride + i]
+    // (structure of arrays, so each of a wave's ten loads is one contiguous run)
     const float4* lobes = nullptr;
+    int lobe_stride = 0;
 };
 }  // namespace lfhd
 using lfhd::GuideHook;
