@@ -28,7 +28,7 @@ constexpr int kGuideParams = 3 * (64 * 64 + 64) + 64 * 40 + 40;  // 15080
 constexpr float kLogKMin = -3.0f;
 constexpr float kLogKMax = 7.0f;
 constexpr float kGuideFrac = 0.5f;   // probability of drawing from the guide
-constexpr int kGuideLobeF4 = 10;     // processed lobes per query: 8 x (mu, kappa), 2 x pi[4]
+constexpr int kGuideLobeF4 = 10;     // processed lobes per query: 8 x (mu, kappa), 2 x (pi * vmf_norm)[4]
 
 LFH_HD int guide_layer_out(int l) { return l == 3 ? kGuideOut : kGuideHid; }
 LFH_HD int guide_layer_offset(int l) { return l * (64 * 64 + 64); }
@@ -93,7 +93,12 @@ LFH_DEV uint16_t to_bf16(float v) {
 }
 LFH_DEV float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-// 40 raw outputs -> 10 float4: lobes (mu, kappa) and the mixture weights.
+// Normalisation of a vMF lobe on the sphere: kappa / (2 pi (1 - e^{-2 kappa})).
+LFH_DEV float vmf_norm(float kappa) { return kappa / (kTwoPi * -expm1f(-2.0f * kappa)); }
+
+// 40 raw outputs -> 10 float4: lobes (mu, kappa), then per lobe its mixture
+// weight times its normalisation, w_k = pi_k * vmf_norm(kappa_k) (the
+// factor every pdf evaluation needs; the sampler divides it out again).
 LFH_DEV void guide_lobes(const float* o, float4* out) {
     float a[kGuideLobes];
     float amax = -3.0e38f;
@@ -118,10 +123,15 @@ LFH_DEV void guide_lobes(const float* o, float4* out) {
         float s = fminf(fmaxf(o[5 * k + 3], kLogKMin), kLogKMax);
         out[k] = make_float4(rx * ir, ry * ir, rz * ir, expf(s));
     }
-    out[8] = make_float4(a[0] * inv, a[1] * inv, a[2] * inv, a[3] * inv);
-    out[9] = make_float4(a[4] * inv, a[5] * inv, a[6] * inv, a[7] * inv);
+    float w[kGuideLobes];
+#pragma unroll
+    for (int k = 0; k < kGuideLobes; ++k) w[k] = a[k] * inv * vmf_norm(out[k].w);
+    out[8] = make_float4(w[0], w[1], w[2], w[3]);
+    out[9] = make_float4(w[4], w[5], w[6], w[7]);
     // a query whose outputs are not all finite gets no guide: weights -1 mark it
     bool ok = isfinite(inv);
+#pragma unroll
+    for (int k = 0; k < kGuideLobes; ++k) ok = ok && isfinite(w[k]);
 #pragma unroll
     for (int k = 0; k < kGuideLobes; ++k)
         ok = ok && isfinite(out[k].x) && isfinite(out[k].y) && isfinite(out[k].z) && isfinite(out[k].w);
@@ -152,17 +162,17 @@ LFH_DEV void guide_store_param(uint16_t* wf, uint16_t* wb, float* bias, int j, f
     }
 }
 
-LFH_DEV float vmf_norm(float kappa) { return kappa / (kTwoPi * -expm1f(-2.0f * kappa)); }
-
+// sum_k pi_k vmf_norm(kappa_k) exp(kappa_k (mu_k . d - 1)), the first product
+// stored per lobe by guide_lobes.
 LFH_DEV float guide_pdf(const float4* __restrict__ lob, V3 d, size_t st = 1) {
     const float4 pa = lob[8 * st], pb = lob[9 * st];
-    const float pis[8] = {pa.x, pa.y, pa.z, pa.w, pb.x, pb.y, pb.z, pb.w};
+    const float w[8] = {pa.x, pa.y, pa.z, pa.w, pb.x, pb.y, pb.z, pb.w};
     float p = 0.0f;
 #pragma unroll
     for (int k = 0; k < kGuideLobes; ++k) {
         const float4 l = lob[k * st];
         const float c = l.x * d.x + l.y * d.y + l.z * d.z;
-        p += pis[k] * vmf_norm(l.w) * expf(l.w * (c - 1.0f));
+        p += w[k] * expf(l.w * (c - 1.0f));
     }
     return p;
 }
@@ -183,7 +193,10 @@ LFH_DEV void guide_frame(V3 n, V3& t, V3& b) {
 // phi = 2 pi u2.
 LFH_DEV V3 guide_sample(const float4* __restrict__ lob, float u0, float u1, float u2, size_t st = 1) {
     const float4 pa = lob[8 * st], pb = lob[9 * st];
-    const float pis[8] = {pa.x, pa.y, pa.z, pa.w, pb.x, pb.y, pb.z, pb.w};
+    const float wn[8] = {pa.x, pa.y, pa.z, pa.w, pb.x, pb.y, pb.z, pb.w};
+    float pis[8];
+#pragma unroll
+    for (int j = 0; j < kGuideLobes; ++j) pis[j] = wn[j] / vmf_norm(lob[j * st].w);
     int k = kGuideLobes - 1;
     float cum = 0.0f;
 #pragma unroll
