@@ -71,6 +71,10 @@ constexpr int kFetchStride = 32;
 constexpr int kFetchWords = kFetchSlices * kFetchStride;
 constexpr int kWorkTotals = 2 * kFetchWords;
 constexpr int kWorkWords = kWorkTotals + 8;
+#ifndef LFH_TRACE_SLICE
+#define LFH_TRACE_SLICE 32   // C1 / C0 / C2: 16: 1557 / 1195 / 326, 24: 1557 / 1226 / 329, 32: 1573 / 1235 / 328, 48: 1558 / 1224 / 317, 96: 1457 / 1233 / 295
+#endif
+constexpr int kTraceSlice = LFH_TRACE_SLICE;  // traversal steps between refills of a wave's idle lanes
 constexpr int kLdsStack = LFH_LDS_STACK;  // LDS entries per lane (power of 2)
 constexpr int kTraceBlock = 256;        // threads per traversal block (4 wave64)
 constexpr uint32_t kSpillStride = 128;  // max stack entries per lane

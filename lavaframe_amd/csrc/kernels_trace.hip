@@ -54,7 +54,7 @@ __global__ __launch_bounds__(kTraceBlock, LFH_TRACE_WAVES) void k_trace(TraceSce
         }
         if (__ballot(active) == 0) break;
         if (active) {
-            bool done = trace_run<ANY, STATS>(sc, s, st, cull, 48, &stats);
+            bool done = trace_run<ANY, STATS>(sc, s, st, cull, kTraceSlice, &stats);
             if (done) {
                 if (st.overflow) atomicOr(err, 1);
                 if (ANY) {
