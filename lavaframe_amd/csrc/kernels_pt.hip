@@ -7,7 +7,7 @@
 //   extend    closest hit (analytic lights first, then the BVH) per queued ray
 //   shade     miss / emitter / surface shading, next-event candidates, BSDF
 //             sampling, Russian roulette; appends continuing rays and shadow
-//             rays with one atomic per wave (ballot + rank)
+//             rays with one atomic per block (ballot + rank, block offsets in LDS)
 //   shadow    any hit per shadow ray; clears the candidate when occluded
 //   accumulate  pending next-event terms + Kahan-compensated accumulation
 // The random-number stream of each path is the reference's: it lives in the
