@@ -233,7 +233,7 @@ LFH_DEV bool light_any(const ShadeScene& sc, V3 o, V3 d, float maxDist) {
 
 template <bool SHADOW, bool MULTI>
 #ifndef LFH_QUEUE_WAVES
-#define LFH_QUEUE_WAVES 1   // queue traversal register budget, waves per SIMD; 1 (88 VGPRs, 5 waves): C0 1226 / C2 308 Msamples/s, C3 0.618; 6 (20 B/lane spilled): 1207 / 323, C3 0.628 (plain and guided both faster)
+#define LFH_QUEUE_WAVES 6   // queue traversal register budget, waves per SIMD; 6 (80 VGPRs, up to 36 B/lane spilled): C0 1216 / C2 388 Msamples/s, 1 (88 VGPRs, 5 waves): 1226 / 368 (2^24-path wavefronts)
 #endif
 __global__ __launch_bounds__(kTraceBlock, LFH_QUEUE_WAVES) void k_trace_queue(ShadeScene sc, const JobDev* __restrict__ jobs,
                                                              const float4* __restrict__ q,
