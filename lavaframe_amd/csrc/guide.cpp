@@ -36,7 +36,7 @@ constexpr int kBatch = 65536;             // samples per optimisation step
 constexpr int kMaxSamples = LFH_GUIDE_SAMPLE_BATCHES * kBatch;  // samples kept from one wavefront
 constexpr int kRecordedPaths = 1 << LFH_GUIDE_RECORD_LOG2;     // recorded paths per wavefront, at most
 #ifndef LFH_GUIDE_WARM_STEPS
-#define LFH_GUIDE_WARM_STEPS 128   // 64: 0.565, 128: 0.564, 256: 0.604 guided/plain time on the guiding room
+#define LFH_GUIDE_WARM_STEPS 128   // 64: 0.565, 128: 0.564, 256: 0.604 guided/plain time on the guiding room (later, with faster passes: 64: 0.937 s, 128: 0.950 s guided)
 #endif
 #ifndef LFH_GUIDE_WARM_REPEATS
 #define LFH_GUIDE_WARM_REPEATS 4
