@@ -14,7 +14,12 @@ ray in, hit out, the six child-box rows and the child references of each
 BVH4 node visited, each triangle and instance record tested), against the
 nominal 8 TB/s and a measured device copy, and the CPU baseline (the oracle's
 FP32 restatement of ClosestHit on a bounded sample of the same rays, on all
-host cores).
+host cores).  The path-tracing legs (Cornell, instanced scene) report
+Msamples/s and Mrays/s, the RMSE of their timed frames against the FP64
+oracle's 65,536-spp references in tests/golden, and their own CPU baselines
+(the oracle's FP32 path tracer on a bounded pixel subset); the serving leg
+reports p50 / p99 job latency against the same stream served one job at a
+time.
 """
 from __future__ import annotations
 
