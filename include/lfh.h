@@ -282,7 +282,7 @@ int  lfh_get_output_ldr(lfh_ctx* ctx, const lfh_postprocess* pp, float inv_sampl
 /* Raw accumulation sum (RGB, w*h*3) and its Kahan compensation (may be NULL). */
 int  lfh_get_accum(lfh_ctx* ctx, float* sum, float* comp, int32_t w, int32_t h);
 /* Render `passes` full passes back to back (each = Update(no reset) + Render
- * over all tiles), graph-captured; for benchmarks and serving. */
+ * over all tiles); for benchmarks and serving. */
 int  lfh_render_passes(lfh_ctx* ctx, int32_t passes);
 /* The preview engine's frame (preview_flareon.glsl; TiledRenderer::Render
  * draws it while the camera moves or instances change, TiledRenderer.cpp:
@@ -366,6 +366,11 @@ int  lfh_server_status(lfh_server* srv, int64_t job_id, lfh_job_status* st);
 /* Result of a finished job: linear HDR (w*h*3) like lfh_get_output_hdr. */
 int  lfh_server_result(lfh_server* srv, int64_t job_id, float* dst, int32_t w, int32_t h);
 int  lfh_server_release(lfh_server* srv, int64_t job_id);
+/* Why a job was rejected (state -1): at submission (invalid scene, over the
+ * budget) or at admission (scene upload or device BVH build failed, buffers
+ * could not be allocated; the server goes on with its other jobs).  "" for
+ * jobs that were not rejected.  Valid until the job is released. */
+const char* lfh_server_job_error(lfh_server* srv, int64_t job_id);
 double lfh_server_clock(const lfh_server* srv);
 
 /* ---- learned path guiding ------------------------------------------------ */
@@ -412,6 +417,14 @@ int  lfh_guide_pdf(lfh_guide* g, const float* inputs9, const float* dirs3, int64
 int  lfh_set_guide(lfh_ctx* ctx, lfh_guide* g, int32_t train_every);
 /* Training samples collected by a ctx's online training so far. */
 int  lfh_guide_samples_seen(const lfh_ctx* ctx, int64_t* n);
+
+/* ---- testing hook ------------------------------------------------------
+ * No reference counterpart.  Makes the next `count` operations at `site` fail
+ * as a device out-of-memory would (site 1: allocation of a render pass's
+ * path state; site 2: scene upload at serving admission) or as a rejected
+ * device TLAS rebuild would (site 3: stack need too deep).  count 0 disarms
+ * the site.  For tests of the error paths; process-wide. */
+int  lfh_debug_fail_next(int32_t site, int32_t count);
 
 #ifdef __cplusplus
 }

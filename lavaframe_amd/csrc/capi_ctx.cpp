@@ -116,37 +116,39 @@ int lfh_set_stream(lfh_ctx* c, void* s) {
 int lfh_synchronize(lfh_ctx* c) {
     if (!c) return ctx_fail(nullptr, LFH_E_INVALID, "lfh_synchronize: NULL ctx");
     HIPCHK(c, hipStreamSynchronize(c->stream), "lfh_synchronize");
-    return LFH_OK;
+    return lfh::check_render_flag(c);
 }
 
 int lfh_update_instances(lfh_ctx* c, const float* transforms, const float* materials, const float* tlas_nodes,
                          int64_t first_node, int64_t count) {
     if (!c) return ctx_fail(nullptr, LFH_E_INVALID, "lfh_update_instances: NULL ctx");
-    if (c->ps.device_build) {
-        // no reference TLAS: the new transforms' TLAS is rebuilt on the device
-        if (tlas_nodes) return ctx_fail(c, LFH_E_INVALID, "lfh_update_instances: a device-built scene takes no TLAS nodes");
-        std::string err = lfh::repack_instances(&c->ps, transforms, nullptr, 0, 0, materials);
-        if (!err.empty()) return ctx_fail(c, LFH_E_INVALID, "lfh_update_instances: " + err);
-        HIPCHK(c, hipStreamSynchronize(c->stream), "lfh_update_instances");
-        err = lfh::device_build_tlas(&c->ps, &c->ds, c->stream);
-        if (!err.empty()) return ctx_fail(c, LFH_E_INVALID, "lfh_update_instances: " + err);
-        auto cpd = [&](DevBuf& b, const std::vector<float>& v) {
-            return hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(float), hipMemcpyHostToDevice, c->stream);
-        };
-        HIPCHK(c, cpd(c->ds.xforms, c->ps.xforms), "lfh_update_instances: transforms");
-        HIPCHK(c, cpd(c->ds.nmats, c->ps.nmats), "lfh_update_instances: normal matrices");
-        HIPCHK(c, cpd(c->ds.materials, c->ps.materials), "lfh_update_instances: materials");
-        HIPCHK(c, hipStreamSynchronize(c->stream), "lfh_update_instances");
-        return LFH_OK;
-    }
-    std::string err = lfh::repack_instances(&c->ps, transforms, tlas_nodes, first_node, count, materials);
-    if (!err.empty()) return ctx_fail(c, LFH_E_INVALID, "lfh_update_instances: " + err);
+    if (c->ps.device_build && tlas_nodes)
+        return ctx_fail(c, LFH_E_INVALID, "lfh_update_instances: a device-built scene takes no TLAS nodes");
+    // A rejected update leaves the scene as it was: the host fields are put
+    // back from this snapshot and the device arrays are not touched (the
+    // device TLAS build restores its own node range on failure).
+    lfh::InstanceSnapshot snap;
+    snap.save(c->ps);
+    auto reject = [&](int code, const std::string& msg) {
+        snap.restore(&c->ps);
+        return ctx_fail(c, code, "lfh_update_instances: " + msg);
+    };
     HIPCHK(c, hipStreamSynchronize(c->stream), "lfh_update_instances");
     auto cp = [&](DevBuf& b, const std::vector<float>& v) {
         return hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(float), hipMemcpyHostToDevice, c->stream);
     };
-    HIPCHK(c, cp(c->ds.nodes, c->ps.nodes4), "lfh_update_instances: nodes");
-    HIPCHK(c, cp(c->ds.insts, c->ps.insts4), "lfh_update_instances: instances");
+    if (c->ps.device_build) {
+        // no reference TLAS: the new transforms' TLAS is rebuilt on the device
+        std::string err = lfh::repack_instances(&c->ps, transforms, nullptr, 0, 0, materials);
+        if (!err.empty()) return reject(LFH_E_INVALID, err);
+        err = lfh::device_build_tlas(&c->ps, &c->ds, c->stream);
+        if (!err.empty()) return reject(LFH_E_INVALID, err);
+    } else {
+        std::string err = lfh::repack_instances(&c->ps, transforms, tlas_nodes, first_node, count, materials);
+        if (!err.empty()) return reject(LFH_E_INVALID, err);
+        HIPCHK(c, cp(c->ds.nodes, c->ps.nodes4), "lfh_update_instances: nodes");
+        HIPCHK(c, cp(c->ds.insts, c->ps.insts4), "lfh_update_instances: instances");
+    }
     HIPCHK(c, cp(c->ds.xforms, c->ps.xforms), "lfh_update_instances: transforms");
     HIPCHK(c, cp(c->ds.nmats, c->ps.nmats), "lfh_update_instances: normal matrices");
     HIPCHK(c, cp(c->ds.materials, c->ps.materials), "lfh_update_instances: materials");

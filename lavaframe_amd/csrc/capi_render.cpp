@@ -69,6 +69,8 @@ int render_init(lfh_ctx* c) {
     if (e == hipSuccess) e = pt_clear4(c->rs.accum.as<float4>(), npix, c->stream);
     if (e == hipSuccess) e = pt_clear4(c->rs.comp.as<float4>(), npix, c->stream);
     if (e == hipSuccess) e = c->pass.alloc(c->rs.npaths);
+    if (e == hipSuccess) e = c->rflag.alloc(64);
+    if (e == hipSuccess) e = hipMemsetAsync(c->rflag.p, 0, 64, c->stream);
     if (e == hipSuccess) {
         c->pt_grid = pt_trace_grid_blocks();
         size_t need = (size_t)c->pt_grid * lfh_trace_threads_per_block() * lfh_trace_spill_stride() * 4;
@@ -80,6 +82,20 @@ int render_init(lfh_ctx* c) {
 }
 
 void render_release(lfh_ctx*) {}
+
+// Reads (and clears) the overflow flag the path-tracing passes set when a
+// traversal stack overflowed; the accumulation is then incomplete.  The
+// scene's stack need is checked at lfh_create, so this is a safety net.
+int check_render_flag(lfh_ctx* c) {
+    int flag = 0;
+    RCHK(c, hipMemcpyAsync(&flag, c->rflag.p, sizeof(int), hipMemcpyDeviceToHost, c->stream), "render flag");
+    RCHK(c, hipStreamSynchronize(c->stream), "render flag");
+    if (flag) {
+        RCHK(c, pt_clear_flag(c->rflag.as<int>(), c->stream), "render flag");
+        return rfail(c, LFH_E_UNSUPPORTED, "traversal stack overflow while rendering (image incomplete)");
+    }
+    return LFH_OK;
+}
 
 // Paths per wavefront targeted by multi-sample passes over one-tile frames
 // (292 B of path state each, 4.9 GB at 2^24).  The last bounces of a
@@ -115,12 +131,13 @@ int render_tile(lfh_ctx* c, int batch) {
     cp.tileY = c->tileY;
     cp.frame = c->frameCounter + c->frameOffset;
     cp.batch = batch;
-    if (batch > 1) {
-        const int64_t need = (int64_t)c->rs.npaths * batch;
+    {
+        // also after a failed allocation, which leaves cap 0
+        const int64_t need = (int64_t)c->rs.npaths * (batch > 1 ? batch : 1);
         if (c->pass.cap < need) {
             hipError_t e = hipStreamSynchronize(c->stream);
             if (e == hipSuccess) e = c->pass.alloc((int)need);
-            if (e != hipSuccess) return rhip(c, e, "render: batch buffers");
+            if (e != hipSuccess) return rhip(c, e, "render: path buffers");
         }
     }
     PassBufs& pb = c->pass;
@@ -137,7 +154,7 @@ int render_tile(lfh_ctx* c, int batch) {
         float4* q[2] = {pb.rq0.as<float4>(), pb.rq1.as<float4>()};
         RCHK(c,
              pt_bounce(sc, depth, pb.cap, q[cur], cnt + cur, q[cur ^ 1], cnt + (cur ^ 1), pb.hq.as<float4>(),
-                       pb.sq.as<float4>(), cnt + 2, lfh::view(pb), c->spill.as<uint32_t>(), work, c->errflag.as<int>(),
+                       pb.sq.as<float4>(), cnt + 2, lfh::view(pb), c->spill.as<uint32_t>(), work, c->rflag.as<int>(),
                        c->cull, c->pt_grid, gh, &gd, c->stream),
              "render: bounce");
     }
@@ -218,7 +235,7 @@ int render_preview(lfh_ctx* c, float scale, int dof, int* pw, int* ph) {
         float4* q[2] = {pb.rq0.as<float4>(), pb.rq1.as<float4>()};
         RCHK(c,
              pt_bounce(sc, depth, pb.cap, q[cur], cnt + cur, q[cur ^ 1], cnt + (cur ^ 1), pb.hq.as<float4>(),
-                       pb.sq.as<float4>(), cnt + 2, lfh::view(pb), c->spill.as<uint32_t>(), work, c->errflag.as<int>(),
+                       pb.sq.as<float4>(), cnt + 2, lfh::view(pb), c->spill.as<uint32_t>(), work, c->rflag.as<int>(),
                        c->cull, c->pt_grid, gh, nullptr, c->stream),
              "lfh_render_preview: bounce");
     }
@@ -356,7 +373,7 @@ int lfh_get_output_hdr(lfh_ctx* c, float* dst, int32_t w, int32_t h) {
          "lfh_get_output_hdr: kernel");
     RCHK(c, hipMemcpyAsync(dst, c->out.p, bytes, hipMemcpyDeviceToHost, c->stream), "lfh_get_output_hdr: copy");
     RCHK(c, hipStreamSynchronize(c->stream), "lfh_get_output_hdr: sync");
-    return LFH_OK;
+    return lfh::check_render_flag(c);
 }
 
 static int get_post(lfh_ctx* c, const lfh_postprocess* pp, float inv, float* rgb, uint8_t* ldr, int32_t w,
@@ -385,7 +402,7 @@ static int get_post(lfh_ctx* c, const lfh_postprocess* pp, float inv, float* rgb
          fn);
     RCHK(c, hipMemcpyAsync(rgb ? (void*)rgb : (void*)ldr, c->out.p, bytes, hipMemcpyDeviceToHost, c->stream), fn);
     RCHK(c, hipStreamSynchronize(c->stream), fn);
-    return LFH_OK;
+    return lfh::check_render_flag(c);
 }
 
 int lfh_get_output_post(lfh_ctx* c, const lfh_postprocess* pp, float inv, float* dst, int32_t w, int32_t h) {
@@ -560,14 +577,14 @@ int lfh_path_trace_rays(lfh_ctx* c, const float* rays6, uint32_t* seeds, int64_t
         float4* q[2] = {pb.rq0.as<float4>(), pb.rq1.as<float4>()};
         RCHK(c,
              pt_bounce(sc, depth, pb.cap, q[cur], cnt + cur, q[cur ^ 1], cnt + (cur ^ 1), pb.hq.as<float4>(),
-                       pb.sq.as<float4>(), cnt + 2, lfh::view(pb), c->spill.as<uint32_t>(), work, c->errflag.as<int>(),
+                       pb.sq.as<float4>(), cnt + 2, lfh::view(pb), c->spill.as<uint32_t>(), work, c->rflag.as<int>(),
                        c->cull, c->pt_grid, gh, nullptr, c->stream),
              fn);
     }
     RCHK(c, pt_collect(lfh::view(pb), (int)n, dout, c->stream), fn);
     RCHK(c, hipMemcpyAsync(radiance, dout, n * 12, hipMemcpyDeviceToHost, c->stream), fn);
     RCHK(c, hipStreamSynchronize(c->stream), fn);
-    return LFH_OK;
+    return lfh::check_render_flag(c);
 }
 
 }  // extern "C"

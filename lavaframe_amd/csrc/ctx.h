@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "../../include/lfh.h"
+#include "capi_common.h"
 #include "dev_common.h"
 #include "pt_types.h"
 
@@ -81,6 +82,28 @@ struct PackedScene {
     int64_t tlas_cap4 = 0;              // BVH4 nodes reserved for the TLAS
 };
 
+// The host fields an instance update rewrites, saved before the update and
+// put back when it is rejected, so a failed update leaves the scene (host and
+// device) as it was.
+struct InstanceSnapshot {
+    std::vector<float> nodes, nodes4, insts, insts4, xforms, nmats, materials;
+    std::vector<int32_t> flat_to_packed, flat_lrleaf;
+    uint32_t root = 0, root4 = 0;
+    int stack_need = 0;
+    void save(const PackedScene& ps) {
+        nodes = ps.nodes; nodes4 = ps.nodes4; insts = ps.insts; insts4 = ps.insts4;
+        xforms = ps.xforms; nmats = ps.nmats; materials = ps.materials;
+        flat_to_packed = ps.flat_to_packed; flat_lrleaf = ps.flat_lrleaf;
+        root = ps.root; root4 = ps.root4; stack_need = ps.stack_need;
+    }
+    void restore(PackedScene* ps) {
+        ps->nodes.swap(nodes); ps->nodes4.swap(nodes4); ps->insts.swap(insts); ps->insts4.swap(insts4);
+        ps->xforms.swap(xforms); ps->nmats.swap(nmats); ps->materials.swap(materials);
+        ps->flat_to_packed.swap(flat_to_packed); ps->flat_lrleaf.swap(flat_lrleaf);
+        ps->root = root; ps->root4 = root4; ps->stack_need = stack_need;
+    }
+};
+
 // Build the device layout from the reference flat arrays; "" on success.
 std::string pack_scene(const lfh_scene_desc& d, PackedScene* out);
 // BVH4 device nodes from the packed BVH2 (called by pack_scene and after
@@ -120,17 +143,16 @@ struct PassBufs {
     DevBuf counters, work;
     DevBuf job;                               // int per path (multi-job passes)
     int cap = 0;
+    // All or nothing: on any failure every buffer is released and cap is 0,
+    // so the next caller that needs paths allocates again instead of
+    // launching on a half-allocated set.
     hipError_t alloc(int n) {
-        hipError_t e;
-        const size_t f4 = 16;
-        DevBuf* b[10] = {&rad, &thr, &absorb, &rng, &misc, &nee, &rq0, &rq1, &hq, &sq};
-        const size_t m[10] = {1, 1, 1, 1, 1, 3, 2, 2, 2, 4};
-        for (int i = 0; i < 10; ++i)
-            if ((e = b[i]->alloc(f4 * m[i] * (size_t)n)) != hipSuccess) return e;
-        if ((e = counters.alloc(64)) != hipSuccess) return e;
-        if ((e = work.alloc(lfhd::kWorkWords * 8)) != hipSuccess) return e;
-        if ((e = hipMemset(work.p, 0, lfhd::kWorkWords * 8)) != hipSuccess) return e;
-        if ((e = job.alloc(4 * (size_t)n)) != hipSuccess) return e;
+        release();
+        hipError_t e = alloc_all(n);
+        if (e != hipSuccess) {
+            release();
+            return e;
+        }
         cap = n;
         return hipSuccess;
     }
@@ -143,8 +165,23 @@ struct PassBufs {
         return rad.bytes + thr.bytes + absorb.bytes + rng.bytes + misc.bytes + nee.bytes + rq0.bytes + rq1.bytes +
                hq.bytes + sq.bytes + counters.bytes + work.bytes + job.bytes;
     }
-};
 
+private:
+    hipError_t alloc_all(int n) {
+        if (lfh_internal::inject_fault(lfh_internal::kFaultPathAlloc)) return hipErrorOutOfMemory;
+        hipError_t e;
+        const size_t f4 = 16;
+        DevBuf* b[10] = {&rad, &thr, &absorb, &rng, &misc, &nee, &rq0, &rq1, &hq, &sq};
+        const size_t m[10] = {1, 1, 1, 1, 1, 3, 2, 2, 2, 4};
+        for (int i = 0; i < 10; ++i)
+            if ((e = b[i]->alloc(f4 * m[i] * (size_t)n)) != hipSuccess) return e;
+        if ((e = counters.alloc(64)) != hipSuccess) return e;
+        if ((e = work.alloc(lfhd::kWorkWords * 8)) != hipSuccess) return e;
+        if ((e = hipMemset(work.p, 0, lfhd::kWorkWords * 8)) != hipSuccess) return e;
+        if ((e = job.alloc(4 * (size_t)n)) != hipSuccess) return e;
+        return hipSuccess;
+    }
+};
 // Per-context buffers of a learned guide attached with lfh_set_guide.
 struct GuideBufs {
     DevBuf lobes;                  // kGuideLobeF4 float4 per hit-queue entry
@@ -177,6 +214,7 @@ struct lfh_ctx {
     int device = 0;
     // traversal scratch
     lfh::DevBuf spill, work, errflag;
+    lfh::DevBuf rflag;            // traversal-overflow flag of the path-tracing passes
     lfh::DevBuf preview;          // float4 per preview pixel (lfh_render_preview)
     int grid_blocks = 0;
     float cull = 1.0f + 1.0f / 1024.0f;

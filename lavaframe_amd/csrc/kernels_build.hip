@@ -662,16 +662,32 @@ std::string device_build_tlas(PackedScene* ps, DeviceScene* ds, hipStream_t s) {
     a.nprim = n;
     a.tlas = 1;
     a.nodes4 = ds->nodes.as<float4>();
+    // The collapse writes the TLAS region of the resident node array in
+    // place; a rebuild that is then rejected (stack too deep) puts the
+    // previous region back, so the scene stays as it was.
+    const size_t region = (size_t)ps->tlas_cap4 * 128;
+    float4* tlas_dev = ds->nodes.as<float4>() + (size_t)ps->blas_nodes4 * 8;
+    DevBuf saved;
+    BCHK(saved.alloc(region), "tlas: alloc");
+    BCHK(hipMemcpyAsync(saved.p, tlas_dev, region, hipMemcpyDeviceToDevice, s), "tlas: save");
+    auto restore = [&]() { (void)hipMemcpyAsync(tlas_dev, saved.p, region, hipMemcpyDeviceToDevice, s);
+                           (void)hipStreamSynchronize(s); };
     std::vector<uint32_t> root;
     int need = 0;
     int64_t written = 0;
     err = collapse(b, a, 1, nullptr, (int)ps->blas_nodes4, ps->tlas_cap4, &root, &need, &written, s);
-    if (!err.empty()) return "tlas " + err;
-    ps->root4 = root[0];
-    ps->stack_need = need + 1 + ps->blas_need;   // TLAS pushes, the BLAS-return marker, the deepest BLAS
-    if (ps->stack_need > (int)kSpillStride)
-        return "BVH too deep for the traversal stack (" + std::to_string(ps->stack_need) + " entries needed, " +
+    if (!err.empty()) {
+        restore();
+        return "tlas " + err;
+    }
+    const int stack_need = need + 1 + ps->blas_need;   // TLAS pushes, the BLAS-return marker, the deepest BLAS
+    if (stack_need > (int)kSpillStride || lfh_internal::inject_fault(lfh_internal::kFaultTlasBuild)) {
+        restore();
+        return "BVH too deep for the traversal stack (" + std::to_string(stack_need) + " entries needed, " +
                std::to_string(kSpillStride) + " available)";
+    }
+    ps->root4 = root[0];
+    ps->stack_need = stack_need;
     device_insts4(ps);
     BCHK(ensure(ds->insts, ps->insts4.size() * 4), "tlas: alloc");
     BCHK(hipMemcpyAsync(ds->insts.p, ps->insts4.data(), ps->insts4.size() * 4, hipMemcpyHostToDevice, s),

@@ -647,6 +647,10 @@ __global__ void k_output(const float4* __restrict__ accum, const float4* __restr
     out[i * 3 + 2] = (s.z - c.z) * inv;
 }
 
+__global__ void k_clear_flag(int* f) {
+    if (threadIdx.x == 0) f[0] = 0;
+}
+
 __global__ void k_clear4(float4* a, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) a[i] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -836,6 +840,11 @@ hipError_t pt_output(const float4* accum, const float4* comp, int64_t npix, floa
 hipError_t pt_clear4(float4* a, int64_t n, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_clear4, dim3(nblk(n)), dim3(256), 0, s, a, n);
+    return hipGetLastError();
+}
+
+hipError_t pt_clear_flag(int* flag, hipStream_t s) {
+    hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(64), 0, s, flag);
     return hipGetLastError();
 }
 

@@ -63,6 +63,7 @@ hipError_t pt_accumulate(const PathBufs& pb, const lfhd::CamParams& cam, float4*
 hipError_t pt_collect(const PathBufs& pb, int n, float* out, hipStream_t s);
 hipError_t pt_output(const float4* accum, const float4* comp, int64_t npix, float inv, float* out, hipStream_t s);
 hipError_t pt_clear4(float4* a, int64_t n, hipStream_t s);
+hipError_t pt_clear_flag(int* flag, hipStream_t s);
 int pt_trace_grid_blocks();
 
 hipError_t stage_eval_bsdf(const lfhd::ShadeScene& sc, const float* fr, const float* L, int n, float* f, float* pdf,

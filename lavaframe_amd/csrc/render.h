@@ -16,4 +16,6 @@ PathBufs view(const PassBufs& b);
 int guide_pass_begin(lfh_ctx* c, int batch, GuideHook* gh, lfhd::GuideDev* gd);
 int guide_pass_end(lfh_ctx* c, const GuideHook& gh, int batch);
 int guide_batch_limit(const lfh_ctx* c);
+// LFH_E_UNSUPPORTED (and the flag cleared) when a pass overflowed a traversal stack.
+int check_render_flag(lfh_ctx* c);
 }  // namespace lfh

@@ -129,6 +129,7 @@ struct Job {
     int frame = 2;  // first pass uses frame 2, like TiledRenderer after Init + Update
     int64_t npix = 0;
     int64_t charged = 0;  // accum + path share held while running
+    std::string error;    // why the job was rejected
 };
 
 }  // namespace
@@ -232,6 +233,12 @@ void lfh_server_destroy(lfh_server* s) {
 
 double lfh_server_clock(const lfh_server* s) { return s ? s->now() : 0.0; }
 
+const char* lfh_server_job_error(lfh_server* s, int64_t id) {
+    if (!s) return "";
+    auto it = s->jobs.find(id);
+    return it == s->jobs.end() ? "unknown job" : it->second->error.c_str();
+}
+
 int lfh_server_submit(lfh_server* s, const lfh_job_desc* jd, int64_t* job_id) {
     if (!s || !jd || !jd->scene) return sfail(s, LFH_E_INVALID, "lfh_server_submit: NULL argument");
     const lfh_options& o = jd->options;
@@ -258,6 +265,7 @@ int lfh_server_submit(lfh_server* s, const lfh_job_desc* jd, int64_t* job_id) {
         std::string perr = lfh::pack_scene(*jd->scene, &e->ps);
         if (!perr.empty()) {
             job->state = -1;
+            job->error = perr;
             if (job_id) *job_id = job->id;
             s->jobs[job->id] = std::move(job);
             return sfail(s, LFH_E_INVALID, "lfh_server_submit: " + perr);
@@ -268,6 +276,7 @@ int lfh_server_submit(lfh_server* s, const lfh_job_desc* jd, int64_t* job_id) {
     if (job_id) *job_id = job->id;
     if (need > s->budget) {
         job->state = -1;
+        job->error = "job exceeds the device budget";
         s->jobs[job->id] = std::move(job);
         return sfail(s, LFH_E_BUDGET, "lfh_server_submit: job needs " + std::to_string(need) +
                                           " device bytes, budget is " + std::to_string(s->budget));
@@ -277,6 +286,18 @@ int lfh_server_submit(lfh_server* s, const lfh_job_desc* jd, int64_t* job_id) {
     e->refs++;
     s->jobs[job->id] = std::move(job);
     return LFH_OK;
+}
+
+// Rejects a queued job at admission: it leaves the queue with its reason and
+// gives up its scene reference.
+static void reject(lfh_server* s, lfh::Job* j, const std::string& why) {
+    j->state = -1;
+    j->error = why;
+    j->t_done = s->now();
+    if (j->scene) {
+        j->scene->refs--;
+        j->scene.reset();
+    }
 }
 
 int lfh_server_step(lfh_server* s, int32_t* active) {
@@ -295,18 +316,33 @@ int lfh_server_step(lfh_server* s, int32_t* active) {
         int64_t need = j->npix * (lfh::kAccumBytes + lfh::kPathBytes) + (e.resident ? 0 : e.bytes);
         if (s->used + need > s->budget) evict_for(s, need);
         if (s->used + need > s->budget) continue;
+        // A job whose scene cannot be made resident, or whose buffers cannot
+        // be allocated, is rejected with its reason; the other jobs go on.
         if (!e.resident) {
             std::string berr;
             hipError_t ue = lfh::upload_scene(e.ps, &e.ds, s->stream, &berr);
-            if (!berr.empty()) return sfail(s, LFH_E_INVALID, "lfh_server_step: device BVH build: " + berr);
-            SCHK(s, ue, "lfh_server_step: scene upload");
+            if (ue == hipSuccess && berr.empty() && lfh_internal::inject_fault(lfh_internal::kFaultSceneUpload))
+                ue = hipErrorOutOfMemory;
+            if (!berr.empty() || ue != hipSuccess) {
+                (void)hipStreamSynchronize(s->stream);
+                e.ds.release();
+                reject(s, j, !berr.empty() ? "device BVH build: " + berr
+                                           : std::string("scene upload: ") + hipGetErrorString(ue));
+                continue;
+            }
             e.resident = true;
             s->used += e.bytes;
         }
-        SCHK(s, j->accum.alloc((size_t)j->npix * 16), "lfh_server_step: accumulation");
-        SCHK(s, j->comp.alloc((size_t)j->npix * 16), "lfh_server_step: accumulation");
-        SCHK(s, pt_clear4(j->accum.as<float4>(), j->npix, s->stream), "lfh_server_step: clear");
-        SCHK(s, pt_clear4(j->comp.as<float4>(), j->npix, s->stream), "lfh_server_step: clear");
+        hipError_t ae = j->accum.alloc((size_t)j->npix * 16);
+        if (ae == hipSuccess) ae = j->comp.alloc((size_t)j->npix * 16);
+        if (ae == hipSuccess) ae = pt_clear4(j->accum.as<float4>(), j->npix, s->stream);
+        if (ae == hipSuccess) ae = pt_clear4(j->comp.as<float4>(), j->npix, s->stream);
+        if (ae != hipSuccess) {
+            j->accum.release();
+            j->comp.release();
+            reject(s, j, std::string("accumulation buffers: ") + hipGetErrorString(ae));
+            continue;
+        }
         j->charged = j->npix * (lfh::kAccumBytes + lfh::kPathBytes);
         s->used += j->charged;
         j->state = 1;
@@ -408,6 +444,13 @@ int lfh_server_step(lfh_server* s, int32_t* active) {
         }
         SCHK(s, pt_accumulate_multi(jt, (int)total, v, s->stream), "lfh_server_step: accumulate");
         SCHK(s, hipStreamSynchronize(s->stream), "lfh_server_step: sync");
+        int flag = 0;
+        SCHK(s, hipMemcpy(&flag, s->errflag.p, sizeof(int), hipMemcpyDeviceToHost), "lfh_server_step: flag");
+        if (flag) {
+            SCHK(s, pt_clear_flag(s->errflag.as<int>(), s->stream), "lfh_server_step: flag");
+            SCHK(s, hipStreamSynchronize(s->stream), "lfh_server_step: flag");
+            return sfail(s, LFH_E_UNSUPPORTED, "lfh_server_step: traversal stack overflow (step not counted)");
+        }
         double t = s->now();
         for (size_t k = 0; k < chosen.size(); ++k) {
             lfh::Job* j = chosen[k];

@@ -166,6 +166,8 @@ _SIGS = {
     "lfh_server_result": (C.c_int, [_P, C.c_int64, f32p, C.c_int32, C.c_int32]),
     "lfh_server_release": (C.c_int, [_P, C.c_int64]),
     "lfh_server_clock": (C.c_double, [_P]),
+    "lfh_server_job_error": (C.c_char_p, [_P, C.c_int64]),
+    "lfh_debug_fail_next": (C.c_int, [C.c_int32, C.c_int32]),
     "lfh_guide_create": (C.c_int, [f32p, f32p, C.c_uint64, C.c_float, C.POINTER(_P)]),
     "lfh_guide_destroy": (None, [_P]),
     "lfh_guide_get_params": (C.c_int, [_P, f32p, i64p]),
@@ -215,6 +217,13 @@ def load(path: str | os.PathLike | None = None):
     if path is None:
         _lib = lib
     return lib
+
+
+def debug_fail_next(site: int, count: int = 1, lib=None):
+    """Testing hook: the next `count` operations at `site` fail (1: render
+    path-state allocation, 2: serving scene upload, 3: device TLAS rebuild)."""
+    lib = lib or load()
+    check(lib.lfh_debug_fail_next(int(site), int(count)), None, lib)
 
 
 class LfhError(RuntimeError):
@@ -864,6 +873,10 @@ class Server:
 
     def release(self, job: int):
         self._chk(self.lib.lfh_server_release(self.h, job))
+
+    def job_error(self, job: int) -> str:
+        """Why a job was rejected ("" if it was not)."""
+        return self.lib.lfh_server_job_error(self.h, job).decode()
 
     def clock(self) -> float:
         return self.lib.lfh_server_clock(self.h)
