@@ -185,7 +185,7 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline and not args.profile_only:
-        cpu = cpu_baseline(s, rays, n_coh, args.cpu_sample)
+        cpu, extra["oracle_parity"] = cpu_baseline(s, rays, n_coh, args.cpu_sample, hits)
     if not args.profile_only and args.legs:
         del d_rays, d_hits, d_occ
         torch.cuda.empty_cache()
@@ -295,17 +295,20 @@ def read_traffic(path):
     return None
 
 
-def cpu_baseline(scene, rays, n_coh, sample):
+def cpu_baseline(scene, rays, n_coh, sample, gpu_hits):
     """The oracle's FP32 restatement of ClosestHit (the reference visit order,
-    no culling) on all host cores over an evenly strided sample of the rays."""
+    no culling) on all host cores over an evenly strided sample of the rays.
+    Also compares its hits with the GPU's on that sample: ids and the bits of
+    t, u, v; rays that differ under the reference's first-found tie rule are
+    traced again with the lowest-index tie rule the GPU follows."""
     try:
         from oracle import native
     except Exception as e:  # pragma: no cover
         log(f"cpu baseline unavailable: {e}")
-        return None
+        return None, None
     if not native.available():
         log("cpu baseline unavailable: oracle not built")
-        return None
+        return None, None
     half = sample // 2
     idx = np.concatenate([np.linspace(0, n_coh - 1, half).astype(np.int64),
                           np.linspace(n_coh, len(rays) - 1, sample - half).astype(np.int64)])
@@ -313,12 +316,31 @@ def cpu_baseline(scene, rays, n_coh, sample):
     orc = native.OracleScene(scene.arrays())
     threads = native.threads()
     t = time.perf_counter()
-    orc.trace_closest(sub, tie_mode=0, nthreads=threads)
+    oh = orc.trace_closest(sub, tie_mode=0, nthreads=threads)
     dt = time.perf_counter() - t
     log(f"cpu baseline: {len(sub)} rays in {dt:.2f}s on {threads} threads")
-    return {"value": round(len(sub) / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"{len(sub)} rays ({half} pinhole + {sample - half} incoherent, evenly strided) of the "
-                      "same 16M-ray workload; FP32 restatement of ClosestHit in the reference visit order"}
+    gh = gpu_hits[idx]
+
+    def differs(a, b):
+        d = np.zeros(len(a), bool)
+        for k in ("prim", "inst", "mat"):
+            d |= a[k] != b[k]
+        for k in ("t", "u", "v"):
+            d |= a[k].view(np.uint32) != b[k].view(np.uint32)
+        return d
+
+    bad0 = np.nonzero(differs(gh, oh))[0]
+    bad1 = bad0
+    if bad0.size:
+        oh1 = orc.trace_closest(np.ascontiguousarray(sub[bad0]), tie_mode=1, nthreads=threads)
+        bad1 = bad0[differs(gh[bad0], oh1)]
+    parity = {"rays_compared": int(len(sub)), "differ_first_found_ties": int(bad0.size),
+              "differ_lowest_index_ties": int(bad1.size), "hits_identical_to_oracle": bool(bad1.size == 0)}
+    log(f"oracle parity on the sample: {parity}")
+    cpu = {"value": round(len(sub) / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+           "sample": f"{len(sub)} rays ({half} pinhole + {sample - half} incoherent, evenly strided) of the "
+                     "same 16M-ray workload; FP32 restatement of ClosestHit in the reference visit order"}
+    return cpu, parity
 
 
 # ---- path-tracing legs ------------------------------------------------------------------

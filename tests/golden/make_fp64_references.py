@@ -7,6 +7,11 @@ against (test infrastructure; about an hour on 8 host cores).
 * instanced_c2_fp64_64k.npz: the instanced scene (64 meshes, 1024 instances,
   1920 x 1080, depth 8), one pixel drawn in each cell of a 32 x 32 grid over the
   frame (seeded), each at 65,536 samples per pixel.
+* aperture_c3_fp64_64k.npz: the guiding room (about 200k triangles, light seen
+  only through a door of 5% of a wall), 1280 x 720, depth 10, with the
+  consistent estimator (options.estimator = 1, which guided rendering uses and
+  which converges to the rendering equation's value), one pixel drawn in each
+  cell of a 32 x 32 grid, each at 65,536 samples per pixel.
 
 Each file holds the per-pixel mean radiance (float32) and the per-sample
 standard deviation (float16), which tests and the benchmark use to tell noise
@@ -62,7 +67,14 @@ def render(sc, px, chunk, label):
 
 
 def main():
-    which = sys.argv[1:] or ["c2", "c0"]
+    which = sys.argv[1:] or ["c2", "c0", "c3"]
+    if "c3" in which:
+        sc = scenes.aperture_room()
+        sc.set_options(estimator=1)
+        px = c2_pixels(1280, 720, seed=7)
+        mean, std = render(sc, px, 64, "c3")
+        np.savez_compressed(OUT / "aperture_c3_fp64_64k.npz", pixels=px, mean=mean.astype(np.float32),
+                            std=std.astype(np.float16), spp=SPP, frame0=FRAME0, estimator=1)
     if "c2" in which:
         px = c2_pixels()
         mean, std = render(scenes.instanced(), px, 64, "c2")

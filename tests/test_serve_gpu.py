@@ -93,3 +93,39 @@ def test_release_and_bad_job():
     srv.release(j)
     with pytest.raises(lfh.LfhError):
         srv.status(j)
+
+
+def test_jobs_arriving_while_others_run_equal_rendering_alone():
+    """Continuous batching: jobs of the Cornell, instanced and guiding-room
+    families are submitted between steps while earlier jobs are still being
+    rendered; every result is bitwise the job rendered alone."""
+    print("[serve] mid-run arrivals", flush=True)
+    fam = [scenes.cornell(64, 48),
+           scenes.instanced(n_meshes=6, n_instances=40, min_tris=100, max_tris=3000, width=56, height=40,
+                            hdr_size=(128, 64), seed=3),
+           scenes.aperture_room(48, 40, max_depth=6, target_tris=8000)]
+    plan = [(fam[0], 200), (fam[1], 150), (fam[2], 130), (fam[0], 90), (fam[1], 70)]
+    alone = []
+    for sc, spp in plan:
+        srv = lfh.Server(max_jobs=8)
+        j = srv.submit_scene(sc, spp)
+        srv.run_until_idle()
+        o = sc.options()
+        alone.append(srv.result(j, o.width, o.height))
+        srv.close()
+    srv = lfh.Server(max_jobs=8)
+    ids = []
+    overlap = 0
+    for sc, spp in plan:                       # one arrival per step
+        ids.append(srv.submit_scene(sc, spp))
+        srv.step()
+        overlap = max(overlap, sum(srv.status(j).state == 1 for j in ids))
+    srv.run_until_idle()
+    assert overlap >= 2                        # later jobs joined running ones
+    for (sc, spp), j, a in zip(plan, ids, alone):
+        st = srv.status(j)
+        assert st.state == 2 and st.spp_done == spp
+        o = sc.options()
+        m = srv.result(j, o.width, o.height)
+        assert np.array_equal(m.view(np.uint32), a.view(np.uint32))
+        assert a.sum() > 0
