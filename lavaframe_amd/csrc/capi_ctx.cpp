@@ -153,6 +153,7 @@ int lfh_update_instances(lfh_ctx* c, const float* transforms, const float* mater
     HIPCHK(c, cp(c->ds.nmats, c->ps.nmats), "lfh_update_instances: normal matrices");
     HIPCHK(c, cp(c->ds.materials, c->ps.materials), "lfh_update_instances: materials");
     HIPCHK(c, hipStreamSynchronize(c->stream), "lfh_update_instances");
+    c->scene_version++;
     return LFH_OK;
 }
 

@@ -4,7 +4,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -32,6 +34,28 @@ static int rhip(lfh_ctx* c, hipError_t e, const char* where) {
     } while (0)
 
 namespace lfh {
+
+// Replays g's captured chain, capturing `launch` first when the key differs.
+hipError_t run_graph(BounceGraph& g, const std::vector<uintptr_t>& key, hipStream_t s,
+                     const std::function<hipError_t()>& launch) {
+    if (!g.exec || g.key != key) {
+        g.reset();
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return e;
+        hipError_t le = launch();
+        e = hipStreamEndCapture(s, &graph);
+        if (le != hipSuccess) e = le;
+        if (e == hipSuccess) e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            g.exec = nullptr;
+            return e;
+        }
+        g.key = key;
+    }
+    return hipGraphLaunch(g.exec, s);
+}
 
 PathBufs view(const PassBufs& b) {
     PathBufs p;
@@ -81,7 +105,15 @@ int render_init(lfh_ctx* c) {
     return LFH_OK;
 }
 
-void render_release(lfh_ctx*) {}
+void render_release(lfh_ctx* c) { c->bgraph.reset(); }
+
+bool bounce_graphs_enabled() {
+    static const bool on = [] {
+        const char* e = std::getenv("LFH_BOUNCE_GRAPHS");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
 
 // Reads (and clears) the overflow flag the path-tracing passes set when a
 // traversal stack overflowed; the accumulation is then incomplete.  The
@@ -149,14 +181,25 @@ int render_tile(lfh_ctx* c, int batch) {
     GuideHook gh{};
     lfhd::GuideDev gd{};
     if (guide_pass_begin(c, batch, &gh, &gd) < 0) return rfail(c, LFH_E_HIP, "render: guide buffers");
-    for (int depth = 0; depth < c->opt.max_depth; ++depth) {
-        const int cur = depth & 1;
-        float4* q[2] = {pb.rq0.as<float4>(), pb.rq1.as<float4>()};
-        RCHK(c,
-             pt_bounce(sc, depth, pb.cap, q[cur], cnt + cur, q[cur ^ 1], cnt + (cur ^ 1), pb.hq.as<float4>(),
-                       pb.sq.as<float4>(), cnt + 2, lfh::view(pb), c->spill.as<uint32_t>(), work, c->rflag.as<int>(),
-                       c->cull, c->pt_grid, gh, &gd, c->stream),
-             "render: bounce");
+    auto chain = [&]() -> hipError_t {
+        for (int depth = 0; depth < c->opt.max_depth; ++depth) {
+            const int cur = depth & 1;
+            float4* q[2] = {pb.rq0.as<float4>(), pb.rq1.as<float4>()};
+            hipError_t e = pt_bounce(sc, depth, pb.cap, q[cur], cnt + cur, q[cur ^ 1], cnt + (cur ^ 1),
+                                     pb.hq.as<float4>(), pb.sq.as<float4>(), cnt + 2, lfh::view(pb),
+                                     c->spill.as<uint32_t>(), work, c->rflag.as<int>(), c->cull, c->pt_grid, gh, &gd,
+                                     c->stream);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    };
+    if (!gh.enabled && bounce_graphs_enabled()) {
+        // the guide hook changes every pass; unguided chains are replayed
+        const std::vector<uintptr_t> key = {(uintptr_t)pb.rad.p, (uintptr_t)pb.cap, (uintptr_t)c->opt.max_depth,
+                                            (uintptr_t)c->stream, (uintptr_t)c->spill.p, (uintptr_t)c->scene_version};
+        RCHK(c, run_graph(c->bgraph, key, c->stream, chain), "render: bounce graph");
+    } else {
+        RCHK(c, chain(), "render: bounce");
     }
     RCHK(c, pt_accumulate(lfh::view(pb), cp, c->rs.accum.as<float4>(), c->rs.comp.as<float4>(), c->opt.kahan, c->stream),
          "render: accumulate");

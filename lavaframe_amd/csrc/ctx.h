@@ -192,6 +192,37 @@ struct GuideBufs {
     int nslots = 0;                // record slots: recorded paths x max_depth, largest wavefront
 };
 
+// The per-depth launch chain of a wavefront (trace, shade, shadow, advance for
+// every depth) captured once as a HIP graph and replayed: its kernels read the
+// queue counters on the device, so the chain is the same for every wavefront
+// of one path capacity, scene, job table and depth.  `key` lists what the
+// chain's arguments depend on; a different key recaptures.
+struct BounceGraph {
+    hipGraphExec_t exec = nullptr;
+    std::vector<uintptr_t> key;
+    BounceGraph() = default;
+    BounceGraph(const BounceGraph&) = delete;
+    BounceGraph& operator=(const BounceGraph&) = delete;
+    ~BounceGraph() { reset(); }
+    void reset() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        exec = nullptr;
+        key.clear();
+    }
+};
+// Bounce graphs are on unless LFH_BOUNCE_GRAPHS=0 (read once).
+bool bounce_graphs_enabled();
+
+// One owner's guided rendering (a context, or a resident scene of the render
+// server): the guide, its training cadence, the passes rendered with it and
+// the buffers its wavefronts use.
+struct GuideRun {
+    struct lfh_guide* guide = nullptr;
+    int train_every = 0;   // every train_every-th pass records training samples; 0: frozen
+    int pass = 0;          // passes rendered with the guide attached
+    GuideBufs buf;
+};
+
 // Uploads the packed scene; for a desc without BVH nodes it also runs the
 // device BVH build (on failure *err, if given, says why).
 hipError_t upload_scene(PackedScene& ps, DeviceScene* ds, hipStream_t s, std::string* err = nullptr);
@@ -232,8 +263,7 @@ struct lfh_ctx {
     int sampleCounter = 1, frameCounter = 1;
     int frameOffset = 0;           // added to the frame number that seeds the RNG
     // guiding
-    struct lfh_guide* guide = nullptr;
-    int guide_train_every = 0;
-    int guide_pass = 0;            // passes rendered with the guide attached
-    lfh::GuideBufs gbuf;
+    lfh::GuideRun grun;
+    lfh::BounceGraph bgraph;       // the unguided bounce chain of render_tile
+    uint64_t scene_version = 0;    // bumped by instance updates (the chain's scene view changes)
 };

@@ -41,6 +41,9 @@ constexpr int kRecordedPaths = 1 << LFH_GUIDE_RECORD_LOG2;     // recorded paths
 #ifndef LFH_GUIDE_WARM_REPEATS
 #define LFH_GUIDE_WARM_REPEATS 4
 #endif
+#ifndef LFH_GUIDE_PLAIN_WARMUP
+#define LFH_GUIDE_PLAIN_WARMUP 0   // > 0: the first passes (one wavefront) sample the BSDF alone and only train the guide
+#endif
 #ifndef LFH_GUIDE_LATE_REPEATS
 #define LFH_GUIDE_LATE_REPEATS 1   // 1: 0.564, 2: 0.638, 4: 0.611 guided/plain time on the guiding room
 #endif
@@ -211,18 +214,27 @@ namespace lfh {
 // Passes that may share one wavefront: one at a time while a training guide is
 // young (each pass then samples the guide the previous pass trained), any
 // number afterwards.
-int guide_batch_limit(const lfh_ctx* c) {
-    const lfh_guide* g = c->guide;
-    if (LFH_GUIDE_SERIAL_WARMUP && g && c->guide_train_every > 0 && g->steps < kWarmSteps) return 1;
+int guide_pass_limit(const GuideRun& run) {
+    const lfh_guide* g = run.guide;
+    if (LFH_GUIDE_PLAIN_WARMUP > 0 && g && run.train_every > 0 && run.pass < LFH_GUIDE_PLAIN_WARMUP)
+        return LFH_GUIDE_PLAIN_WARMUP - run.pass;   // the warm-up passes share one wavefront
+    if (LFH_GUIDE_SERIAL_WARMUP && g && run.train_every > 0 && g->steps < kWarmSteps) return 1;
     return std::numeric_limits<int>::max();
 }
 
 namespace {
-// Training passes among the wavefront's passes guide_pass .. guide_pass + batch - 1.
-int training_passes(const lfh_ctx* c, int batch) {
-    if (c->guide_train_every <= 0 || c->opt.max_depth <= 0) return 0;
+// Whether a wavefront starting at run.pass belongs to the BSDF-only warm-up.
+bool plain_warmup(const GuideRun& run) {
+    return LFH_GUIDE_PLAIN_WARMUP > 0 && run.train_every > 0 && run.pass < LFH_GUIDE_PLAIN_WARMUP;
+}
+}  // namespace
+
+namespace {
+// Training passes among the wavefront's passes run.pass .. run.pass + batch - 1.
+int training_passes(const GuideRun& run, int max_depth, int batch) {
+    if (run.train_every <= 0 || max_depth <= 0) return 0;
     int n = 0;
-    for (int i = 0; i < batch; ++i) n += ((c->guide_pass + i) % c->guide_train_every) == 0;
+    for (int i = 0; i < batch; ++i) n += ((run.pass + i) % run.train_every) == 0;
     return n;
 }
 
@@ -231,64 +243,70 @@ struct RecordPlan {
     int64_t paths;
     int stride, recorded, nslots;
 };
-RecordPlan record_plan(const lfh_ctx* c, int batch) {
+RecordPlan record_plan(const GuidePassArgs& a) {
     RecordPlan r;
-    r.paths = (int64_t)c->rs.npaths * batch;
+    r.paths = a.paths;
     r.stride = (int)std::max<int64_t>(1, (r.paths + kRecordedPaths - 1) / kRecordedPaths);
     r.recorded = (int)((r.paths + r.stride - 1) / r.stride);
-    r.nslots = r.recorded * std::max(1, c->opt.max_depth);
+    r.nslots = r.recorded * std::max(1, a.max_depth);
     return r;
 }
 }  // namespace
 
-int guide_pass_begin(lfh_ctx* c, int batch, GuideHook* gh, lfhd::GuideDev* gd) {
+int guide_begin(const GuidePassArgs& a, int batch, GuideHook* gh, lfhd::GuideDev* gd) {
     *gh = GuideHook{};
-    lfh_guide* g = c->guide;
+    GuideRun& run = *a.run;
+    lfh_guide* g = run.guide;
     if (!g) return 0;
-    GuideBufs& b = c->gbuf;
+    GuideBufs& b = a.buf ? *a.buf : run.buf;
     // lobes per hit-queue entry: the queues grow with multi-pass wavefronts
-    const size_t lobe_bytes = (size_t)c->pass.cap * kGuideLobeF4 * sizeof(float4);
+    const size_t lobe_bytes = (size_t)a.cap * kGuideLobeF4 * sizeof(float4);
     if (b.lobes.bytes < lobe_bytes) {
-        if (hipStreamSynchronize(c->stream) != hipSuccess || b.lobes.alloc(lobe_bytes) != hipSuccess) return -1;
+        if (hipStreamSynchronize(a.stream) != hipSuccess || b.lobes.alloc(lobe_bytes) != hipSuccess) return -1;
     }
     gh->enabled = 1;
+    gh->sample = plain_warmup(run) ? 0 : 1;
     gh->lobes = b.lobes.as<const float4>();
-    gh->lobe_stride = c->pass.cap;
-    gh->max_depth = c->opt.max_depth;
+    gh->lobe_stride = a.cap;
+    gh->max_depth = a.max_depth;
     *gd = g->dev();
-    if (training_passes(c, batch) > 0) {
-        const RecordPlan rp = record_plan(c, batch);
+    if (training_passes(run, a.max_depth, batch) > 0) {
+        const RecordPlan rp = record_plan(a);
+        if (rp.nslots > b.nslots) return -1;
         gh->train = 1;
         gh->stride = rp.stride;
-        gh->phase = (c->guide_pass / c->guide_train_every) % rp.stride;
+        gh->phase = (run.pass / run.train_every) % rp.stride;
         gh->rec = b.rec.as<float4>();
-        if (pt_clear4(b.rec.as<float4>(), (int64_t)rp.nslots * 4, c->stream) != hipSuccess) return -1;
+        if (pt_clear4(b.rec.as<float4>(), (int64_t)rp.nslots * 4, a.stream) != hipSuccess) return -1;
     }
     return 1;
 }
 
-int guide_pass_end(lfh_ctx* c, const GuideHook& gh, int batch) {
-    lfh_guide* g = c->guide;
-    const int ntrain = training_passes(c, batch);
-    c->guide_pass += batch;
+int guide_end(const GuidePassArgs& a, const GuideHook& gh, int batch) {
+    GuideRun& run = *a.run;
+    lfh_guide* g = run.guide;
+    int ntrain = training_passes(run, a.max_depth, batch);
+    const bool warm_wavefront = plain_warmup(run);
+    run.pass += batch;
     if (!g || !gh.train) return LFH_OK;
-    GuideBufs& b = c->gbuf;
-    const RecordPlan rp = record_plan(c, batch);
+    GuideBufs& b = a.buf ? *a.buf : run.buf;
+    const RecordPlan rp = record_plan(a);
     // collect, then train on the device-side sample count: no host round trip
-    hipError_t e = lfhd::guide_collect(b.rec.as<float4>(), rp.nslots, c->opt.max_depth, gh.stride, gh.phase,
-                                       c->pass.rad.as<float4>(), (int)rp.paths, b.block_counts.as<int>(),
-                                       b.block_offsets.as<int>(), b.total.as<int>(), b.samples.as<float4>(),
-                                       kMaxSamples, b.block_wsum.as<float>(), b.total.as<float>() + 1,
-                                       reinterpret_cast<unsigned long long*>(b.total.as<char>() + 8), c->stream);
+    hipError_t e = lfhd::guide_collect(b.rec.as<float4>(), rp.nslots, a.max_depth, gh.stride, gh.phase, a.rad,
+                                       (int)rp.paths, b.block_counts.as<int>(), b.block_offsets.as<int>(),
+                                       b.total.as<int>(), b.samples.as<float4>(), kMaxSamples,
+                                       b.block_wsum.as<float>(), b.total.as<float>() + 1,
+                                       reinterpret_cast<unsigned long long*>(b.total.as<char>() + 8), a.stream);
     if (e != hipSuccess) return ghip(g, e, "guide: collect");
     // one Adam step per training pass on a strided batch of the wavefront's
     // samples; several while the guide is young
+    if (warm_wavefront) ntrain = 1;   // one round of the warm-up steps on the BSDF-sampled passes' samples
     for (int t = 0; t < ntrain; ++t) {
-        const int reps = g->steps < kWarmSteps ? kWarmRepeats : kLateRepeats;
+        const int reps = warm_wavefront ? kWarmSteps : (g->steps < kWarmSteps ? kWarmRepeats : kLateRepeats);
         for (int r = 0; r < reps; ++r) {
             lfhd::GuideAdam ad = g->adam(true);
             e = lfhd::guide_train(g->dev(), b.samples.as<float4>(), b.total.as<int>(), (int)(g->steps & 0x7fffffff),
-                                  kBatch, b.partial.as<float>(), ad, nullptr, nullptr, c->stream,
+                                  kBatch, b.partial.as<float>(), ad, nullptr, nullptr, a.stream,
                                   b.total.as<float>() + 1, kWeightCap);
             if (e != hipSuccess) return ghip(g, e, "guide: train");
             g->steps++;
@@ -296,6 +314,75 @@ int guide_pass_end(lfh_ctx* c, const GuideHook& gh, int batch) {
     }
     return LFH_OK;
 }
+
+// Buffers guided wavefronts of up to `cap` paths and paths of up to
+// max_depth vertices need; the counters are cleared when first allocated.
+hipError_t guide_buffers(GuideBufs& b, int cap, int max_depth, hipStream_t s) {
+    // records for the largest wavefront: kRecordedPaths paths x max_depth
+    b.nslots = std::max(b.nslots, kRecordedPaths * (max_depth > 0 ? max_depth : 1));
+    const int nblocks = lfhd::guide_collect_blocks(b.nslots);
+    hipError_t e = ensure(b.lobes, (size_t)cap * kGuideLobeF4 * sizeof(float4));
+    if (e == hipSuccess) e = ensure(b.rec, (size_t)b.nslots * 4 * sizeof(float4));
+    if (e == hipSuccess) e = ensure(b.samples, (size_t)kMaxSamples * 4 * sizeof(float4));
+    if (e == hipSuccess) e = ensure(b.block_counts, (size_t)(nblocks + 1) * sizeof(int));
+    if (e == hipSuccess) e = ensure(b.block_offsets, (size_t)(nblocks + 1) * sizeof(int));
+    if (e == hipSuccess) e = ensure(b.block_wsum, (size_t)(nblocks + 1) * sizeof(float));
+    if (e == hipSuccess && !b.total.p) {
+        e = b.total.alloc(64);   // int count, float weight sum, u64 running count at byte 8
+        if (e == hipSuccess) e = hipMemsetAsync(b.total.p, 0, 64, s);
+    }
+    if (e == hipSuccess) e = ensure(b.partial, lfhd::guide_partial_floats(kBatch) * sizeof(float));
+    return e;
+}
+
+size_t guide_buffer_bytes(int cap, int max_depth) {
+    const size_t nslots = (size_t)kRecordedPaths * (max_depth > 0 ? max_depth : 1);
+    const size_t nblocks = (size_t)lfhd::guide_collect_blocks((int)nslots) + 1;
+    return (size_t)cap * kGuideLobeF4 * sizeof(float4) + nslots * 4 * sizeof(float4) +
+           (size_t)kMaxSamples * 4 * sizeof(float4) + nblocks * 12 + 64 +
+           lfhd::guide_partial_floats(kBatch) * sizeof(float);
+}
+
+hipError_t guide_attach(GuideRun& run, lfh_guide* g, int train_every, int cap, int max_depth, hipStream_t s) {
+    run.guide = g;
+    run.train_every = train_every;
+    if (!g) return hipSuccess;
+    hipError_t e = guide_buffers(run.buf, cap, max_depth, s);
+    if (e != hipSuccess) run.guide = nullptr;
+    return e;
+}
+
+int guide_samples_seen(const GuideRun& run, hipStream_t s, int64_t* n) {
+    unsigned long long v = 0;
+    if (run.buf.total.p) {
+        if (hipMemcpyAsync(&v, run.buf.total.as<char>() + 8, sizeof v, hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            return LFH_E_HIP;
+    }
+    *n = (int64_t)v;
+    return LFH_OK;
+}
+
+
+// The context's hooks: its path buffers and its one-tile wavefronts.
+namespace {
+GuidePassArgs ctx_args(lfh_ctx* c, int batch) {
+    GuidePassArgs a;
+    a.run = &c->grun;
+    a.paths = (int64_t)c->rs.npaths * batch;
+    a.cap = c->pass.cap;
+    a.max_depth = c->opt.max_depth;
+    a.rad = c->pass.rad.as<float4>();
+    a.stream = c->stream;
+    return a;
+}
+}  // namespace
+
+int guide_pass_begin(lfh_ctx* c, int batch, GuideHook* gh, lfhd::GuideDev* gd) {
+    return guide_begin(ctx_args(c, batch), batch, gh, gd);
+}
+int guide_pass_end(lfh_ctx* c, const GuideHook& gh, int batch) { return guide_end(ctx_args(c, batch), gh, batch); }
+int guide_batch_limit(const lfh_ctx* c) { return guide_pass_limit(c->grun); }
 
 }  // namespace lfh
 
@@ -459,41 +546,15 @@ int lfh_set_guide(lfh_ctx* c, lfh_guide* g, int32_t train_every) {
     if (g && c->opt.estimator != 1)
         return gfail(g, LFH_E_STATE, "lfh_set_guide: guided sampling needs options.estimator = 1");
     if (hipStreamSynchronize(c->stream) != hipSuccess) return gfail(g, LFH_E_HIP, "lfh_set_guide: sync");
-    c->guide = g;
-    c->guide_train_every = train_every;
-    if (!g) return LFH_OK;
-    lfh::GuideBufs& b = c->gbuf;
-    // records for the largest wavefront: kRecordedPaths paths x max_depth
-    b.nslots = kRecordedPaths * (c->opt.max_depth > 0 ? c->opt.max_depth : 1);
-    const int nblocks = lfhd::guide_collect_blocks(b.nslots);
-    hipError_t e = ensure(b.lobes, (size_t)c->pass.cap * kGuideLobeF4 * sizeof(float4));
-    if (e == hipSuccess) e = ensure(b.rec, (size_t)b.nslots * 4 * sizeof(float4));
-    if (e == hipSuccess) e = ensure(b.samples, (size_t)kMaxSamples * 4 * sizeof(float4));
-    if (e == hipSuccess) e = ensure(b.block_counts, (size_t)(nblocks + 1) * sizeof(int));
-    if (e == hipSuccess) e = ensure(b.block_offsets, (size_t)(nblocks + 1) * sizeof(int));
-    if (e == hipSuccess) e = ensure(b.block_wsum, (size_t)(nblocks + 1) * sizeof(float));
-    if (e == hipSuccess && !b.total.p) {
-        e = b.total.alloc(64);   // int count, float weight sum, u64 running count at byte 8
-        if (e == hipSuccess) e = hipMemsetAsync(b.total.p, 0, 64, c->stream);
-    }
-    if (e == hipSuccess) e = ensure(b.partial, lfhd::guide_partial_floats(kBatch) * sizeof(float));
-    if (e != hipSuccess) {
-        c->guide = nullptr;
-        return ghip(g, e, "lfh_set_guide");
-    }
+    hipError_t e = lfh::guide_attach(c->grun, g, train_every, c->pass.cap, c->opt.max_depth, c->stream);
+    if (e != hipSuccess) return ghip(g, e, "lfh_set_guide");
     return LFH_OK;
 }
 
 int lfh_guide_samples_seen(const lfh_ctx* c, int64_t* n) {
     if (!c || !n) return gfail(nullptr, LFH_E_INVALID, "lfh_guide_samples_seen: NULL argument");
-    unsigned long long v = 0;
-    if (c->gbuf.total.p) {
-        if (hipMemcpyAsync(&v, c->gbuf.total.as<char>() + 8, sizeof v, hipMemcpyDeviceToHost, c->stream) !=
-                hipSuccess ||
-            hipStreamSynchronize(c->stream) != hipSuccess)
-            return gfail(nullptr, LFH_E_HIP, "lfh_guide_samples_seen: copy");
-    }
-    *n = (int64_t)v;
+    if (lfh::guide_samples_seen(c->grun, c->stream, n) != LFH_OK)
+        return gfail(nullptr, LFH_E_HIP, "lfh_guide_samples_seen: copy");
     return LFH_OK;
 }
 

