@@ -330,7 +330,31 @@ void BvhTranslator::UpdateTLAS(const Bvh* tlas, const std::vector<MeshInstance>&
 // (the OBJ reader the reference links, tiny_obj_loader.h:866-997): integer
 // digits m = 10 m + d in double, fraction digits m += d * 10^-k from a table of
 // double constants, exponent via ldexp(m * 5^e, e).  Kept alike so vertex
-// positions are bit-identical to the reference mesh arrays.
+// positions are bit-identical to the reference mesh arrays.  It follows
+// tinyobjloader's tryParseDouble step for step and is distributed under that
+// library's licence:
+//
+//   The MIT License (MIT)
+//   Copyright (c) 2012-Present, Syoyo Fujita and many contributors.
+//
+//   Permission is hereby granted, free of charge, to any person obtaining a
+//   copy of this software and associated documentation files (the
+//   "Software"), to deal in the Software without restriction, including
+//   without limitation the rights to use, copy, modify, merge, publish,
+//   distribute, sublicense, and/or sell copies of the Software, and to permit
+//   persons to whom the Software is furnished to do so, subject to the
+//   following conditions:
+//
+//   The above copyright notice and this permission notice shall be included
+//   in all copies or substantial portions of the Software.
+//
+//   THE SOFTWARE IS PROVIDED "AS IS", WITHOUT WARRANTY OF ANY KIND, EXPRESS
+//   OR IMPLIED, INCLUDING BUT NOT LIMITED TO THE WARRANTIES OF
+//   MERCHANTABILITY, FITNESS FOR A PARTICULAR PURPOSE AND NONINFRINGEMENT. IN
+//   NO EVENT SHALL THE AUTHORS OR COPYRIGHT HOLDERS BE LIABLE FOR ANY CLAIM,
+//   DAMAGES OR OTHER LIABILITY, WHETHER IN AN ACTION OF CONTRACT, TORT OR
+//   OTHERWISE, ARISING FROM, OUT OF OR IN CONNECTION WITH THE SOFTWARE OR THE
+//   USE OR OTHER DEALINGS IN THE SOFTWARE.
 double ParseObjReal(const char** token) {
     const char* s = *token;
     s += std::strspn(s, " \t");
