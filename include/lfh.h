@@ -330,6 +330,19 @@ int  lfh_env_pdf(lfh_ctx* ctx, const float* dir, int64_t n, float* pdf);
 int  lfh_path_trace_rays(lfh_ctx* ctx, const float* rays6, uint32_t* seeds, int64_t n,
                          float* radiance);
 
+/* lfh_path_trace_rays on device arrays with no device allocation in the
+ * call.  No counterpart in the reference: it is for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  d_rays6: n x 6 floats (origin, direction); d_seeds: n x 4
+ * uint32 pcg4d states (read only); d_radiance: n x 3 floats; all device
+ * pointers, asynchronous on the ctx stream.  The path state lives in a static
+ * device array of about 19 MiB per GPU (held until the process ends), so n is
+ * at most 65536 (larger n: LFH_E_INVALID); every call on one GPU shares that
+ * array, so calls that overlap on one GPU (other streams or host threads)
+ * race.  Radiance is bitwise that of lfh_path_trace_rays. */
+int  lfh_path_trace_rays_device_noalloc(lfh_ctx* ctx, const float* d_rays6, const uint32_t* d_seeds, int64_t n,
+                                        float* d_radiance);
+
 /* ---- serving: continuous-batching render service ------------------------ */
 typedef struct lfh_server lfh_server;
 typedef struct lfh_job_desc {

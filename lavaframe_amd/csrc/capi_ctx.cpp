@@ -61,6 +61,7 @@ int lfh_create(const lfh_scene_desc* desc, const lfh_options* opt, lfh_ctx** out
         return ctx_fail(nullptr, LFH_E_INVALID, "lfh_create: " + err);
     }
     hipError_t e = hipGetDevice(&c->device);
+    if (e == hipSuccess) e = lfh::noalloc_setup(c->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         int r = hip_fail(nullptr, e, "lfh_create: stream");

@@ -40,4 +40,7 @@ size_t guide_buffer_bytes(int cap, int max_depth);
 int guide_samples_seen(const GuideRun& run, hipStream_t s, int64_t* n);
 // LFH_E_UNSUPPORTED (and the flag cleared) when a pass overflowed a traversal stack.
 int check_render_flag(lfh_ctx* c);
+// Device address of the no-alloc entry point's scratch array on `device`,
+// taken once per device (called by lfh_create with the device current).
+hipError_t noalloc_setup(int device);
 }  // namespace lfh

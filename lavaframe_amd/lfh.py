@@ -158,6 +158,7 @@ _SIGS = {
     "lfh_sample_env": (C.c_int, [_P, u32p, C.c_int64, f32p, f32p, f32p]),
     "lfh_env_pdf": (C.c_int, [_P, f32p, C.c_int64, f32p]),
     "lfh_path_trace_rays": (C.c_int, [_P, f32p, u32p, C.c_int64, f32p]),
+    "lfh_path_trace_rays_device_noalloc": (C.c_int, [_P, _P, _P, C.c_int64, _P]),
     "lfh_server_create": (C.c_int, [C.c_int64, C.c_int32, C.POINTER(_P)]),
     "lfh_server_destroy": (None, [_P]),
     "lfh_server_submit": (C.c_int, [_P, C.POINTER(JobDesc), i64p]),
