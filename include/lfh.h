@@ -148,7 +148,10 @@ typedef struct lfh_options {
                                       * 1: consistent MIS, the BSDF strategy weighted by
                                       *    its lobe-mixture pdf (disney.glsl:289-290) after
                                       *    BSDF samples too (unbiased; used with guiding) */
-    int32_t reserved[5];
+    int32_t frame_offset;            /* added to the frame number that seeds each pass's
+                                      * random numbers (0 = the reference's sequence, >= 0);
+                                      * see lfh_set_frame_offset.  No reference counterpart */
+    int32_t reserved[4];
 } lfh_options;
 
 /* Post-process options: the RenderOptions fields behind postprocess.glsl's

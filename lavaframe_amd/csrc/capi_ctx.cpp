@@ -52,9 +52,11 @@ int lfh_create(const lfh_scene_desc* desc, const lfh_options* opt, lfh_ctx** out
         return ctx_fail(nullptr, LFH_E_INVALID, "lfh_create: bad resolution or depth");
     if (opt->estimator != 0 && opt->estimator != 1)
         return ctx_fail(nullptr, LFH_E_INVALID, "lfh_create: estimator must be 0 or 1");
+    if (opt->frame_offset < 0) return ctx_fail(nullptr, LFH_E_INVALID, "lfh_create: negative frame offset");
     lfh_ctx* c = new (std::nothrow) lfh_ctx;
     if (!c) return ctx_fail(nullptr, LFH_E_OOM, "lfh_create: out of host memory");
     c->opt = *opt;
+    c->frameOffset = opt->frame_offset;
     std::string err = lfh::pack_scene(*desc, &c->ps);
     if (!err.empty()) {
         delete c;

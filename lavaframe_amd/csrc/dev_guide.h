@@ -33,10 +33,14 @@ constexpr int kGuideLobeF4 = 10;     // processed lobes per query: 8 x (mu, kapp
 LFH_HD int guide_layer_out(int l) { return l == 3 ? kGuideOut : kGuideHid; }
 LFH_HD int guide_layer_offset(int l) { return l * (64 * 64 + 64); }
 
-// Device view of the guide's weights.  wf: per layer [64 out][64 in] BF16 (the
-// forward product's B operand, zero rows past the layer's outputs); wb: per
-// layer [64 in][64 out] BF16 (the backward product's B operand); bias: per
-// layer 64 FP32 (zero-padded).
+// Device view of the guide's weights.  wf holds two BF16 images of the
+// weights: [0, kGuideWImage) per layer [64 out][64 in] (the training forward's
+// B operand, zero rows past the layer's outputs) and [kGuideWImage,
+// 2 kGuideWImage) the inference image (guide_infer_frag); wb: per layer
+// [64 in][64 out] BF16 (the backward product's B operand); bias: per layer 64
+// FP32 in output order (zero-padded), then the same in inference row order.
+constexpr int kGuideWImage = 4 * 4096;
+constexpr int kGuideBImage = 4 * 64;
 struct GuideDev {
     const uint16_t* wf;
     const uint16_t* wb;
@@ -51,16 +55,26 @@ LFH_HD float guide_norm_coord(float p, float lo, float inv_ext) {
 }
 
 #if defined(__HIP__)
+// sin and cos of pi v for v in [-1, 1]: the hardware's v_sin_f32 / v_cos_f32
+// take the angle in revolutions (sin 2 pi r), here r = v / 2 (exact).
+LFH_DEV void sincos_pi(float v, float* s, float* c) {
+    const float r = 0.5f * v;
+    *s = __builtin_amdgcn_sinf(r);
+    *c = __builtin_amdgcn_cosf(r);
+}
+
 // The 64 features of one query (x already normalised).  Each coordinate's
-// base angle pi*c gets one sincos; the higher octaves follow by the double-
-// angle identities sin 2a = 2 sin a cos a, cos 2a = (cos a - sin a)(cos a + sin a),
-// which keeps the absolute error below 1e-5 at the top octave (2^5).
+// base angle pi*c gets one hardware sin/cos pair; the higher octaves follow
+// by the double-angle identities sin 2a = 2 sin a cos a, cos 2a = (cos a -
+// sin a)(cos a + sin a), which multiply the base error by at most 2^5 at the
+// top octave: a few 1e-5 absolute, far below the BF16 step (2^-8 near 1) the
+// features are rounded to.
 LFH_DEV void guide_encode(V3 x, V3 n, V3 wo, float* f) {
     const float xs[3] = {x.x, x.y, x.z}, ns[3] = {n.x, n.y, n.z}, ws[3] = {wo.x, wo.y, wo.z};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         float sn, cs;
-        sincosf(kPi * xs[a], &sn, &cs);
+        sincos_pi(xs[a], &sn, &cs);
 #pragma unroll
         for (int fr = 0; fr < 6; ++fr) {
             f[(a * 6 + fr) * 2] = sn;
@@ -69,12 +83,12 @@ LFH_DEV void guide_encode(V3 x, V3 n, V3 wo, float* f) {
             sn = s2;
             cs = c2;
         }
-        sincosf(kPi * ns[a], &sn, &cs);
+        sincos_pi(ns[a], &sn, &cs);
         f[36 + a * 4 + 0] = sn;
         f[36 + a * 4 + 1] = cs;
         f[36 + a * 4 + 2] = 2.0f * sn * cs;
         f[36 + a * 4 + 3] = (cs - sn) * (cs + sn);
-        sincosf(kPi * ws[a], &sn, &cs);
+        sincos_pi(ws[a], &sn, &cs);
         f[48 + a * 4 + 0] = sn;
         f[48 + a * 4 + 1] = cs;
         f[48 + a * 4 + 2] = 2.0f * sn * cs;
@@ -94,60 +108,62 @@ LFH_DEV uint16_t to_bf16(float v) {
 LFH_DEV float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 // Normalisation of a vMF lobe on the sphere: kappa / (2 pi (1 - e^{-2 kappa})).
-LFH_DEV float vmf_norm(float kappa) { return kappa / (kTwoPi * -expm1f(-2.0f * kappa)); }
+// Every kappa here is exp(s) with s >= -3, so 1 - e^{-2 kappa} >= 0.095 and
+// the subtraction loses under 1e-6 relative: hardware exp and reciprocal.
+LFH_DEV float vmf_norm(float kappa) {
+    return kappa * __builtin_amdgcn_rcpf(kTwoPi * (1.0f - __expf(-2.0f * kappa)));
+}
 
-// 40 raw outputs -> 10 float4: lobes (mu, kappa), then per lobe its mixture
-// weight times its normalisation, w_k = pi_k * vmf_norm(kappa_k) (the
-// factor every pdf evaluation needs; the sampler divides it out again).
-LFH_DEV void guide_lobes(const float* o, float4* out) {
-    float a[kGuideLobes];
-    float amax = -3.0e38f;
-#pragma unroll
-    for (int k = 0; k < kGuideLobes; ++k) {
-        a[k] = o[5 * k + 4];
-        amax = fmaxf(amax, a[k]);
-    }
-    float sum = 0.0f;
-#pragma unroll
-    for (int k = 0; k < kGuideLobes; ++k) {
-        a[k] = expf(a[k] - amax);
-        sum += a[k];
-    }
-    const float inv = 1.0f / sum;
-#pragma unroll
-    for (int k = 0; k < kGuideLobes; ++k) {
-        float rx = o[5 * k], ry = o[5 * k + 1], rz = o[5 * k + 2];
-        float rn = sqrtf(rx * rx + ry * ry + rz * rz);
-        float ir = rn > 1e-12f ? 1.0f / rn : 0.0f;
-        if (rn <= 1e-12f) rz = 1.0f, ir = 1.0f;
-        float s = fminf(fmaxf(o[5 * k + 3], kLogKMin), kLogKMax);
-        out[k] = make_float4(rx * ir, ry * ir, rz * ir, expf(s));
-    }
-    float w[kGuideLobes];
-#pragma unroll
-    for (int k = 0; k < kGuideLobes; ++k) w[k] = a[k] * inv * vmf_norm(out[k].w);
-    out[8] = make_float4(w[0], w[1], w[2], w[3]);
-    out[9] = make_float4(w[4], w[5], w[6], w[7]);
-    // a query whose outputs are not all finite gets no guide: weights -1 mark it
-    bool ok = isfinite(inv);
-#pragma unroll
-    for (int k = 0; k < kGuideLobes; ++k) ok = ok && isfinite(w[k]);
-#pragma unroll
-    for (int k = 0; k < kGuideLobes; ++k)
-        ok = ok && isfinite(out[k].x) && isfinite(out[k].y) && isfinite(out[k].z) && isfinite(out[k].w);
-    if (!ok) {
-        out[8] = make_float4(-1.f, -1.f, -1.f, -1.f);
-        out[9] = make_float4(-1.f, -1.f, -1.f, -1.f);
-    }
+// Processed lobes, kGuideLobeF4 float4 per query: float4 k < 8 is lobe k's
+// (mu, kappa) from outputs 5k..5k+4 = (r, s, a), mu = normalize(r), kappa =
+// exp(clamp(s, kLogKMin, kLogKMax)); float4 8 and 9 hold w_k = pi_k *
+// vmf_norm(kappa_k), pi = softmax(a) (the factor every pdf evaluation needs;
+// the sampler divides it out again), or -1 for a query whose outputs are not
+// all finite (no guide).  guide_lobe makes one (mu, kappa).
+LFH_DEV float4 guide_lobe(const float* o5) {
+    float rx = o5[0], ry = o5[1], rz = o5[2];
+    const float r2 = rx * rx + ry * ry + rz * rz;
+    float ir = r2 > 1e-24f ? __builtin_amdgcn_rsqf(r2) : 0.0f;
+    if (r2 <= 1e-24f) rz = 1.0f, ir = 1.0f;
+    const float sk = fminf(fmaxf(o5[3], kLogKMin), kLogKMax);
+    return make_float4(rx * ir, ry * ir, rz * ir, __expf(sk));
 }
 
 // The processed lobes of one query are lob[k * st], k = 0..9 (st = 1: ten
 // consecutive float4; larger st: one query of a structure-of-arrays buffer).
-// True when they are usable (see guide_lobes).
+// True when they are usable.
 LFH_DEV bool guide_valid(const float4* __restrict__ lob, size_t st = 1) { return lob[8 * st].x >= 0.0f; }
 
-// Stores master parameter j (flat layout) into the BF16 operand copies:
-// wf[layer][out][in], wb[layer][in][out], or the FP32 bias.
+// Inference image.  Inference runs the MLP transposed, Y^T = W^T X^T, on
+// 16-query tiles with v_mfma_f32_16x16x32_bf16 (weights = A operand, 16 rows
+// x 32 K per fragment; activations = B operand, 32 K x 16 queries): a layer's
+// accumulators then hold, in lane group g = lane / 16, rows 4g .. 4g+3 of each
+// 16-row tile for query lane % 16.  The image orders rows and columns so that
+// no activation leaves its lane between layers:
+//   row p of layers 0-2 is hidden unit p; row p of layer 3 is output
+//   10 (q / 4) + 4 t + q % 4 (q = p % 16, t = p / 16; none where 4 t + q % 4
+//   >= 10), so lane group g ends with lobes 2g and 2g+1 of its query;
+//   column k = 32 ks + 8 g + j of layer 0 is input feature k; of layers 1-3,
+//   hidden unit 16 (2 ks + j / 4) + 4 g + j % 4: the B fragment a lane packs
+//   from its own accumulators of tiles 2 ks and 2 ks + 1.
+LFH_HD int guide_infer_row(int layer, int out) {
+    if (layer < 3) return out;
+    const int g = out / 10, s = out - 10 * g;
+    return (s >> 2) * 16 + g * 4 + (s & 3);
+}
+LFH_HD int guide_infer_col(int layer, int in) {
+    if (layer == 0) return in;
+    const int t = in >> 4, g = (in >> 2) & 3;
+    return (t >> 1) * 32 + g * 8 + (t & 1) * 4 + (in & 3);
+}
+// Element (row p, column k) of a layer's image: A fragments stored
+// [layer][tile p/16][K step k/32][lane 16 ((k % 32) / 8) + p % 16][k % 8].
+LFH_HD int guide_infer_frag(int layer, int p, int k) {
+    return (((layer * 4 + (p >> 4)) * 2 + (k >> 5)) * 64 + ((k & 31) >> 3) * 16 + (p & 15)) * 8 + (k & 7);
+}
+
+// Stores master parameter j (flat layout) into the BF16 operand copies (both
+// images of wf, and wb) or the FP32 biases (both orders).
 LFH_DEV void guide_store_param(uint16_t* wf, uint16_t* wb, float* bias, int j, float w) {
     const int layer = j < 3 * (64 * 64 + 64) ? j / (64 * 64 + 64) : 3;
     const int r = j - guide_layer_offset(layer);
@@ -156,14 +172,17 @@ LFH_DEV void guide_store_param(uint16_t* wf, uint16_t* wb, float* bias, int j, f
         const int in = r / nout, out = r - in * nout;
         const uint16_t h = to_bf16(w);
         wf[layer * 4096 + out * 64 + in] = h;
+        wf[kGuideWImage + guide_infer_frag(layer, guide_infer_row(layer, out), guide_infer_col(layer, in))] = h;
         wb[layer * 4096 + in * 64 + out] = h;
     } else {
-        bias[layer * 64 + (r - 64 * nout)] = w;
+        const int out = r - 64 * nout;
+        bias[layer * 64 + out] = w;
+        bias[kGuideBImage + layer * 64 + guide_infer_row(layer, out)] = w;
     }
 }
 
 // sum_k pi_k vmf_norm(kappa_k) exp(kappa_k (mu_k . d - 1)), the first product
-// stored per lobe by guide_lobes.
+// stored per lobe in the processed lobes.
 LFH_DEV float guide_pdf(const float4* __restrict__ lob, V3 d, size_t st = 1) {
     const float4 pa = lob[8 * st], pb = lob[9 * st];
     const float w[8] = {pa.x, pa.y, pa.z, pa.w, pb.x, pb.y, pb.z, pb.w};

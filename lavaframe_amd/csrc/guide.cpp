@@ -28,7 +28,7 @@ using lfhd::kGuideParams;
 #define LFH_GUIDE_SAMPLE_BATCHES 4
 #endif
 #ifndef LFH_GUIDE_SERIAL_WARMUP
-#define LFH_GUIDE_SERIAL_WARMUP 1
+#define LFH_GUIDE_SERIAL_WARMUP 0
 #endif
 
 namespace {
@@ -42,7 +42,12 @@ constexpr int kRecordedPaths = 1 << LFH_GUIDE_RECORD_LOG2;     // recorded paths
 #define LFH_GUIDE_WARM_REPEATS 4
 #endif
 #ifndef LFH_GUIDE_PLAIN_WARMUP
-#define LFH_GUIDE_PLAIN_WARMUP 0   // > 0: the first passes (one wavefront) sample the BSDF alone and only train the guide
+// > 0: the first passes (one wavefront) sample the BSDF alone and only train
+// the guide; 0: serial warm-up (one pass per wavefront until kWarmSteps).
+// Served guiding room, guided / plain time to RMSE 1e-2 / 5e-3 / 2.5e-3
+// (single runs): serial 1.10 / 0.84 / 0.63; 4 passes 0.63 / 0.74 / 0.86;
+// 8: 0.74 / 0.82 / 0.65; 16: 0.79 / 0.72 / 0.93.
+#define LFH_GUIDE_PLAIN_WARMUP 4
 #endif
 #ifndef LFH_GUIDE_LATE_REPEATS
 #define LFH_GUIDE_LATE_REPEATS 1   // 1: 0.564, 2: 0.638, 4: 0.611 guided/plain time on the guiding room
@@ -50,7 +55,10 @@ constexpr int kRecordedPaths = 1 << LFH_GUIDE_RECORD_LOG2;     // recorded paths
 constexpr int kWarmSteps = LFH_GUIDE_WARM_STEPS;       // Adam steps taken at the warm-up rate
 constexpr int kLateRepeats = LFH_GUIDE_LATE_REPEATS;   // Adam steps per training pass after the warm-up
 constexpr int kWarmRepeats = LFH_GUIDE_WARM_REPEATS;   // optimisation passes over a pass's samples while warming up
-constexpr float kWeightCap = 32.0f;       // online training: weights / their mean, capped
+#ifndef LFH_GUIDE_WEIGHT_CAP
+#define LFH_GUIDE_WEIGHT_CAP 32.0f
+#endif
+constexpr float kWeightCap = LFH_GUIDE_WEIGHT_CAP;   // online training: weights / their mean, capped
 }  // namespace
 
 struct lfh_guide {
@@ -404,9 +412,9 @@ int lfh_guide_create(const float* scene_min3, const float* scene_max3, uint64_t 
     if (e == hipSuccess) e = g->master.alloc(kGuideParams * sizeof(float));
     if (e == hipSuccess) e = g->m.alloc(kGuideParams * sizeof(float));
     if (e == hipSuccess) e = g->v.alloc(kGuideParams * sizeof(float));
-    if (e == hipSuccess) e = g->wf.alloc(4 * 4096 * sizeof(uint16_t));
+    if (e == hipSuccess) e = g->wf.alloc(2 * lfhd::kGuideWImage * sizeof(uint16_t));
     if (e == hipSuccess) e = g->wb.alloc(4 * 4096 * sizeof(uint16_t));
-    if (e == hipSuccess) e = g->bias.alloc(4 * 64 * sizeof(float));
+    if (e == hipSuccess) e = g->bias.alloc(2 * lfhd::kGuideBImage * sizeof(float));
     if (e == hipSuccess) e = g->count.alloc(64);
     if (e == hipSuccess) e = g->grad.alloc((kGuideParams + 1) * sizeof(float));
     if (e == hipSuccess) e = g->loss.alloc(64);

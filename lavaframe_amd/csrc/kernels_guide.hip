@@ -6,10 +6,14 @@
 //   A (16 x 32):  row l%16, k = (l/16)*8 .. +7     (8 BF16 = 16 B per lane)
 //   B (32 x 16):  col l%16, k = (l/16)*8 .. +7
 //   D (16 x 16):  col l%16, rows (l/16)*4 .. +3    (4 FP32)
-// Activations live in LDS as BF16 rows of kPitch elements (64 + 8 padding so
-// that the 16 B fragment reads of 16 consecutive rows spread over the banks);
-// weights are read as B fragments straight from global memory (32 KiB per
-// direction, resident in L1/L2).
+// Training: activations live in LDS as BF16 rows of kPitch elements (64 + 8
+// padding so that the 16 B fragment reads of 16 consecutive rows spread over
+// the banks); weights are read as B fragments straight from global memory
+// (32 KiB per direction, resident in L1/L2).  Inference: the encoded inputs
+// pass through LDS once, then the MLP runs transposed with the weights held in
+// registers and the activations never leaving their lanes (k_guide_infer).
+#include <algorithm>
+
 #include "dev_guide.h"
 #include "dev_shade.h"
 #include "guide_kernels.h"
@@ -19,11 +23,15 @@ namespace lfhd {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef short i16x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kPitch = 72;            // BF16 elements per activation row
 constexpr int kOutPitch = 41;         // FP32 elements per output row
-constexpr int kInferWaves = 2;        // waves per inference block
-constexpr int kInferBlock = 64 * kInferWaves;
+constexpr int kInferBlock = 256;      // 4 persistent waves
+constexpr int kInferWaves = kInferBlock / 64;
 constexpr int kTrainRows = 64;        // samples per training block
 constexpr int kTrainBlock = 256;      // 4 waves: one 16-row M tile each
 constexpr int kPartStride = kGuideParams + 1;  // per-block partial: grads + loss
@@ -32,6 +40,11 @@ LFH_DEV bf16x8 ld_frag(const uint16_t* p) { return *reinterpret_cast<const bf16x
 
 LFH_DEV f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
 
+// Two FP32 as a BF16 pair, round to nearest even (v_cvt_pk_bf16_f32).
+LFH_DEV uint32_t pack_bf16x2(float a, float b) {
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){a, b}, bf16x2));
+}
+
 // Writes the 64 encoded features of one query as BF16 into an LDS row.
 LFH_DEV void encode_row(uint16_t* row, const GuideDev& g, V3 p, V3 n, V3 wo) {
     const V3 x = v3(guide_norm_coord(p.x, g.lo[0], g.inv_ext[0]), guide_norm_coord(p.y, g.lo[1], g.inv_ext[1]),
@@ -39,67 +52,10 @@ LFH_DEV void encode_row(uint16_t* row, const GuideDev& g, V3 p, V3 n, V3 wo) {
     float f[kGuideIn];
     guide_encode(x, n, wo, f);
 #pragma unroll
-    for (int k = 0; k < kGuideIn; k += 2)
-        *reinterpret_cast<uint32_t*>(row + k) = (uint32_t)to_bf16(f[k]) | ((uint32_t)to_bf16(f[k + 1]) << 16);
-}
-
-// Forward pass of the 64 rows of X owned by this wave (rows r0 .. r0+63 in
-// 16-row tiles), in place: hidden layers overwrite X, the 40 outputs go to O
-// (FP32, kOutPitch).  Each 16-row tile reads only its own rows, so the
-// in-place update is safe within the wave.
-// O may overlay X (one buffer per wave): the output layer takes the row tiles
-// last to first, and O's rows of tile mt (byte offsets from mt * 16 *
-// kOutPitch * 4 on) then lie at or above the activations of the tiles not yet
-// read (below mt * 16 * kPitch * 2), while tile mt's own activations are in
-// registers before its outputs are stored.
-LFH_DEV void mlp_forward_rows(const GuideDev& g, uint16_t* X, float* O, int ntiles) {
-    const int lane = threadIdx.x & 63;
-    const int lr = lane & 15, lk = (lane >> 4) * 8, lq = (lane >> 4) * 4;
-#pragma unroll
-    for (int layer = 0; layer < kGuideLayers; ++layer) {
-        const uint16_t* W = g.wf + layer * 4096;
-        const float* bias = g.bias + layer * 64;
-        constexpr int kN[4] = {4, 4, 4, 3};
-        const int nN = kN[layer];
-        // the layer's B fragments and biases once per wave, reused by every row tile
-        bf16x8 bw[4][2];
-        float bb[4];
-#pragma unroll
-        for (int nt = 0; nt < 4; ++nt) {
-            if (nt >= nN) continue;
-            bb[nt] = bias[nt * 16 + lr];
-#pragma unroll
-            for (int ks = 0; ks < 2; ++ks) bw[nt][ks] = ld_frag(W + (nt * 16 + lr) * 64 + ks * 32 + lk);
-        }
-        for (int it = 0; it < ntiles; ++it) {
-            const int mt = layer < kGuideLayers - 1 ? it : ntiles - 1 - it;
-            f32x4 acc[4];
-#pragma unroll
-            for (int nt = 0; nt < 4; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int ks = 0; ks < 2; ++ks) {
-                const bf16x8 a = ld_frag(X + (mt * 16 + lr) * kPitch + ks * 32 + lk);
-#pragma unroll
-                for (int nt = 0; nt < 4; ++nt)
-                    if (nt < nN) acc[nt] = mfma(a, bw[nt][ks], acc[nt]);
-            }
-            // the tile's activation reads stay before its stores even where
-            // O (float) overlays X (uint16_t)
-            __asm__ volatile("" ::: "memory");
-#pragma unroll
-            for (int nt = 0; nt < 4; ++nt) {
-                if (nt >= nN) continue;
-                const int col = nt * 16 + lr;
-                const float b = bb[nt];
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const int row = mt * 16 + lq + j;
-                    const float v = acc[nt][j] + b;
-                    if (layer < 3) X[row * kPitch + col] = to_bf16(fmaxf(v, 0.0f));
-                    else if (col < kGuideOut) O[row * kOutPitch + col] = v;
-                }
-            }
-        }
+    for (int k = 0; k < kGuideIn; k += 8) {
+        const u32x4 v = {pack_bf16x2(f[k], f[k + 1]), pack_bf16x2(f[k + 2], f[k + 3]), pack_bf16x2(f[k + 4], f[k + 5]),
+                         pack_bf16x2(f[k + 6], f[k + 7])};
+        *reinterpret_cast<u32x4*>(row + k) = v;
     }
 }
 
@@ -153,55 +109,136 @@ LFH_DEV void record_query(const GuideHook& gh, int path, int depth, V3 pos, V3 n
     r[2] = make_float4(wo.x, wo.y, wo.z, -1.0f);
 }
 
+// Accumulators of tiles 2ks and 2ks+1 after bias -> BF16 (v_cvt_pk_bf16_f32,
+// round to nearest even) -> ReLU, as the next layer's B fragment of K step ks.
+// The ReLU is a max with 0 of the BF16 pairs read as signed 16-bit integers:
+// a BF16 with its sign bit set is a negative integer.
+
+LFH_DEV uint32_t pack_relu2(float a, float b) {
+    const i16x2 h = __builtin_bit_cast(i16x2, pack_bf16x2(a, b));
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(h, (i16x2){0, 0}));
+}
+LFH_DEV bf16x8 next_operand(f32x4 lo, f32x4 hi) {
+    const u32x4 u = {pack_relu2(lo[0], lo[1]), pack_relu2(lo[2], lo[3]), pack_relu2(hi[0], hi[1]),
+                     pack_relu2(hi[2], hi[3])};
+    return __builtin_bit_cast(bf16x8, u);
+}
+
+// Persistent waves, 64 queries a round: lane l fetches and encodes query
+// base + l into the wave's LDS rows; then each tile of 16 queries runs the
+// four layers on the inference image (weights as A operand, loaded into
+// registers once per wave), the accumulators of one layer packed in place
+// into the next layer's B operand.  Lane group g = lane / 16 ends with lobes
+// 2g, 2g+1 of query lane % 16 of the tile; the softmax over the 8 lobes and
+// the validity flag are reduced over the 4 lane groups by shuffles.  Output:
+// raw40 (40 outputs per query) and/or the processed lobes (dev_guide.h;
+// QUEUE: structure of arrays with stride gh.lobe_stride).
 template <bool QUEUE>
 __global__ __launch_bounds__(kInferBlock) void k_guide_infer(GuideDev g, const float* __restrict__ in9, int n_host,
                                                              InferQueue q, float* __restrict__ raw40,
                                                              float4* __restrict__ lobes) {
-    // one buffer per wave: BF16 activations, then FP32 outputs over them
-    constexpr int kWaveBytes = 64 * kOutPitch * 4 > 64 * kPitch * 2 ? 64 * kOutPitch * 4 : 64 * kPitch * 2;
-    __shared__ __attribute__((aligned(16))) uint8_t sU[kInferWaves][kWaveBytes];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __shared__ __attribute__((aligned(16))) uint16_t sX[kInferWaves][64 * kPitch];
+    __shared__ float4 sBias[kGuideBImage / 4];
     const uint32_t n = QUEUE ? *q.count : (uint32_t)n_host;
     if ((uint32_t)blockIdx.x * kInferBlock >= n) return;  // block-uniform
-    const uint32_t i = (uint32_t)blockIdx.x * kInferBlock + threadIdx.x;
-    const bool live = i < n;
-    V3 pos = v3(0.f, 0.f, 0.f), nrm = v3(0.f, 0.f, 1.f), wo = v3(0.f, 0.f, 1.f);
-    if (live) {
-        if (QUEUE) {
-            int path;
-            const bool surf = queue_query(q, i, pos, nrm, wo, path);
-            const GuideHook& gh = q.gh;
-            if (surf && gh.train && (path % gh.stride) == gh.phase) record_query(gh, path, q.depth, pos, nrm, wo);
-        } else {
-            const float* s = in9 + (size_t)i * 9;
-            pos = v3(s[0], s[1], s[2]);
-            nrm = v3(s[3], s[4], s[5]);
-            wo = v3(s[6], s[7], s[8]);
-        }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int lr = lane & 15, grp = lane >> 4;
+    if (threadIdx.x < kGuideBImage / 4)
+        sBias[threadIdx.x] = reinterpret_cast<const float4*>(g.bias + kGuideBImage)[threadIdx.x];
+    bf16x8 w[kGuideLayers][4][2];
+    {
+        const uint16_t* wi = g.wf + kGuideWImage + lane * 8;
+#pragma unroll
+        for (int l = 0; l < kGuideLayers; ++l)
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+                for (int ks = 0; ks < 2; ++ks)
+                    w[l][nt][ks] = (l < 3 || nt < 3) ? ld_frag(wi + ((l * 4 + nt) * 2 + ks) * 512) : bf16x8{};
     }
-    uint16_t* X = reinterpret_cast<uint16_t*>(sU[wave]);
-    float* O = reinterpret_cast<float*>(sU[wave]);
-    encode_row(X + lane * kPitch, g, pos, nrm, wo);
     __syncthreads();
-    mlp_forward_rows(g, X, O, 4);
-    __syncthreads();
-    if (live) {
-        const float* o = O + lane * kOutPitch;
-        if (raw40) {
-            float* dst = raw40 + (size_t)i * kGuideOut;
-            for (int k = 0; k < kGuideOut; ++k) dst[k] = o[k];
+    uint16_t* X = sX[wave];
+    const uint32_t step = gridDim.x * kInferBlock;
+    for (uint32_t base = (blockIdx.x * kInferWaves + wave) * 64u; base < n; base += step) {
+        const uint32_t i = base + lane;
+        V3 pos = v3(0.f, 0.f, 0.f), nrm = v3(0.f, 0.f, 1.f), wo = v3(0.f, 0.f, 1.f);
+        if (i < n) {
+            if (QUEUE) {
+                int path;
+                const bool surf = queue_query(q, i, pos, nrm, wo, path);
+                const GuideHook& gh = q.gh;
+                if (surf && gh.train && (path % gh.stride) == gh.phase) record_query(gh, path, q.depth, pos, nrm, wo);
+            } else {
+                const float* s = in9 + (size_t)i * 9;
+                pos = v3(s[0], s[1], s[2]);
+                nrm = v3(s[3], s[4], s[5]);
+                wo = v3(s[6], s[7], s[8]);
+            }
         }
-        if (lobes) {
-            float ov[kGuideOut];
+        // the rows are read back by other lanes of this wave only: LDS
+        // operations of one wave complete in order
+        encode_row(X + lane * kPitch, g, pos, nrm, wo);
+#pragma unroll 1
+        for (int t = 0; t < 4; ++t) {
+            if (base + t * 16 >= n) break;   // wave-uniform
+            const uint16_t* xr = X + (t * 16 + lr) * kPitch + grp * 8;
+            bf16x8 b0 = ld_frag(xr), b1 = ld_frag(xr + 32);
+            f32x4 acc[4];
 #pragma unroll
-            for (int k = 0; k < kGuideOut; ++k) ov[k] = o[k];
-            float4 lb[kGuideLobeF4];
-            guide_lobes(ov, lb);
-            // the hit queue's lobes in structure-of-arrays order, host queries' packed
-            const size_t st = QUEUE ? (size_t)q.gh.lobe_stride : 1;
-            float4* dst = lobes + (QUEUE ? (size_t)i : (size_t)i * kGuideLobeF4);
+            for (int l = 0; l < kGuideLayers; ++l) {
+                const int nN = l < 3 ? 4 : 3;
 #pragma unroll
-            for (int k = 0; k < kGuideLobeF4; ++k) dst[k * st] = lb[k];
+                for (int nt = 0; nt < 4; ++nt) {
+                    if (nt >= nN) continue;
+                    const float4 bb = sBias[(l * 64 + nt * 16 + grp * 4) >> 2];
+                    acc[nt] = f32x4{bb.x, bb.y, bb.z, bb.w};
+                    acc[nt] = mfma(w[l][nt][0], b0, acc[nt]);
+                    acc[nt] = mfma(w[l][nt][1], b1, acc[nt]);
+                }
+                if (l < 3) {
+                    b0 = next_operand(acc[0], acc[1]);
+                    b1 = next_operand(acc[2], acc[3]);
+                }
+            }
+            // outputs 10 grp + s, s = 4 nt + i, in acc[nt][i]
+            float o[10];
+#pragma unroll
+            for (int s2 = 0; s2 < 10; ++s2) o[s2] = acc[s2 >> 2][s2 & 3];
+            const uint32_t qi = base + t * 16 + lr;
+            if (raw40 && qi < n) {
+                float* dst = raw40 + (size_t)qi * kGuideOut + grp * 10;
+#pragma unroll
+                for (int s2 = 0; s2 < 10; ++s2) dst[s2] = o[s2];
+            }
+            if (!lobes) continue;
+            // the processed lobes (dev_guide.h) over the query's four lane
+            // groups: lobes 2 grp, 2 grp + 1 here
+            float m = fmaxf(o[4], o[9]);
+            m = fmaxf(m, __shfl_xor(m, 16));
+            m = fmaxf(m, __shfl_xor(m, 32));
+            const float e0 = __expf(o[4] - m), e1 = __expf(o[9] - m);
+            float sum = e0 + e1;
+            sum += __shfl_xor(sum, 16);
+            sum += __shfl_xor(sum, 32);
+            const float inv = __builtin_amdgcn_rcpf(sum);
+            const float4 lb[2] = {guide_lobe(o), guide_lobe(o + 5)};
+            const float wk0 = e0 * inv * vmf_norm(lb[0].w), wk1 = e1 * inv * vmf_norm(lb[1].w);
+            float wk[2] = {wk0, wk1};
+            bool ok = isfinite(inv) && isfinite(wk0) && isfinite(wk1);
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                ok = ok && isfinite(lb[j].x) && isfinite(lb[j].y) && isfinite(lb[j].z) && isfinite(lb[j].w);
+            int okw = ok ? 1 : 0;
+            okw &= __shfl_xor(okw, 16);
+            okw &= __shfl_xor(okw, 32);
+            if (!okw) wk[0] = wk[1] = -1.0f;   // a query with a non-finite output gets no guide
+            if (qi < n) {
+                const size_t st = QUEUE ? (size_t)q.gh.lobe_stride : 1;
+                float4* dst = lobes + (QUEUE ? (size_t)qi : (size_t)qi * kGuideLobeF4);
+                dst[(2 * grp) * st] = lb[0];
+                dst[(2 * grp + 1) * st] = lb[1];
+                reinterpret_cast<float2*>(dst + (8 + (grp >> 1)) * st)[grp & 1] = make_float2(wk[0], wk[1]);
+            }
         }
     }
 }
@@ -653,11 +690,28 @@ __global__ __launch_bounds__(256) void k_guide_compact(CollectArgs c) {
 // ---- launchers ------------------------------------------------------------------------
 static inline int nblk(int64_t n, int b) { return (int)((n + b - 1) / b); }
 
+// Blocks of a persistent inference launch for n queries: what the chip holds
+// at once, at most one block per 256 queries.
+static int infer_blocks(int64_t n) {
+    static int resident = 0;
+    if (resident == 0) {
+        int dev = 0, per_cu = 0, cus = 256;
+        hipDeviceProp_t prop;
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+            cus = prop.multiProcessorCount;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_guide_infer<true>, kInferBlock, 0) !=
+                hipSuccess || per_cu <= 0)
+            per_cu = 2;
+        resident = per_cu * cus;
+    }
+    return (int)std::min<int64_t>(resident, nblk(n, kInferBlock));
+}
+
 hipError_t guide_forward_host(const GuideDev& g, const float* in9, int n, float* raw40, float4* lobes,
                               hipStream_t s) {
     if (n <= 0) return hipSuccess;
     InferQueue q{};
-    hipLaunchKernelGGL(k_guide_infer<false>, dim3(nblk(n, kInferBlock)), dim3(kInferBlock), 0, s, g, in9, n, q, raw40,
+    hipLaunchKernelGGL(k_guide_infer<false>, dim3(infer_blocks(n)), dim3(kInferBlock), 0, s, g, in9, n, q, raw40,
                        lobes);
     return hipGetLastError();
 }
@@ -667,7 +721,7 @@ hipError_t guide_infer_queue(const GuideDev& g, const ShadeScene& sc, const floa
                              hipStream_t s) {
     if (maxq <= 0) return hipSuccess;
     InferQueue q{sc, rayq, hitq, count, gh, depth};
-    hipLaunchKernelGGL(k_guide_infer<true>, dim3(nblk(maxq, kInferBlock)), dim3(kInferBlock), 0, s, g,
+    hipLaunchKernelGGL(k_guide_infer<true>, dim3(infer_blocks(maxq)), dim3(kInferBlock), 0, s, g,
                        (const float*)nullptr, 0, q, (float*)nullptr, lobes);
     return hipGetLastError();
 }

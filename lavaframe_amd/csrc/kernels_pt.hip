@@ -338,7 +338,7 @@ LFH_DEV void combine_nee(PathBufs& pb, int p, V3& rad) {
 // Shading class of hit-queue entry i (miss, light, then the material's
 // dominant lobe group), for binning a block's entries before shading.
 #ifndef LFH_SHADE_BIN
-#define LFH_SHADE_BIN 1   // 1: a block shades its entries grouped by class (the order within a class kept)
+#define LFH_SHADE_BIN 0   // 1: a block shades its entries grouped by class (C2: 388.1 binned vs 390.6 Msamples/s; C0 unchanged)
 #endif
 constexpr int kShadeClasses = 6;
 template <bool MULTI>

@@ -67,40 +67,73 @@ constexpr int64_t kStepMaxPasses = 64;    // passes of one job per step, at most
 #ifndef LFH_SERVE_MAX_OFFSET
 #define LFH_SERVE_MAX_OFFSET 0.1
 #endif
+// 1: the step's path budget is shared fairly between the running jobs (each
+// offered an equal part, what a job does not need going to the others); 0:
+// earliest deadline first takes what it needs before the next.
+#ifndef LFH_SERVE_FAIR
+#define LFH_SERVE_FAIR 1
+#endif
 constexpr double kDeadlineSecondsPerSample = 1.0 / LFH_SERVE_RATE;
 constexpr double kDeadlineMaxOffset = LFH_SERVE_MAX_OFFSET;
 
-uint64_t fnv(uint64_t h, const void* p, size_t n) {
+// Content hash of the scene arrays (identical scenes share one resident
+// copy).  Each array: 32-byte blocks into four independent multiply-rotate
+// lanes (the round shape of xxHash64, here with splitmix64's odd constants),
+// the tail bytes zero-padded into a last block, then the lanes, the length
+// and the running hash through splitmix64's finaliser.  Several GB/s on one
+// core, so a submit of a 20 MB scene spends a few milliseconds hashing.
+uint64_t fmix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+inline uint64_t rotl64(uint64_t x, int r) { return (x << r) | (x >> (64 - r)); }
+inline uint64_t lane_round(uint64_t acc, uint64_t w) {
+    return rotl64(acc + w * 0xBF58476D1CE4E5B9ull, 31) * 0x94D049BB133111EBull;
+}
+uint64_t content_hash(uint64_t h, const void* p, size_t n) {
     const unsigned char* c = static_cast<const unsigned char*>(p);
-    for (size_t i = 0; i < n; ++i) {
-        h ^= c[i];
-        h *= 1099511628211ull;
+    uint64_t v[4] = {0x9E3779B97F4A7C15ull, 0x3C6EF372FE94F82Aull, 0xDAA66D2C7DDF743Full, 0x78DDE6E5FD29F054ull};
+    size_t i = 0;
+    for (; i + 32 <= n; i += 32) {
+        uint64_t w[4];
+        std::memcpy(w, c + i, 32);
+        v[0] = lane_round(v[0], w[0]);
+        v[1] = lane_round(v[1], w[1]);
+        v[2] = lane_round(v[2], w[2]);
+        v[3] = lane_round(v[3], w[3]);
     }
-    return h;
+    if (i < n) {
+        uint64_t w[4] = {0, 0, 0, 0};
+        std::memcpy(w, c + i, n - i);
+        for (int k = 0; k < 4; ++k) v[k] = lane_round(v[k], w[k]);
+    }
+    const uint64_t x = rotl64(v[0], 1) + rotl64(v[1], 7) + rotl64(v[2], 12) + rotl64(v[3], 18);
+    return fmix64(h ^ fmix64(x ^ (uint64_t)n));
 }
 
 uint64_t hash_desc(const lfh_scene_desc& d) {
     uint64_t h = 1469598103934665603ull;
-    h = fnv(h, d.nodes, (size_t)d.num_nodes * 36);
-    h = fnv(h, &d.top_level_index, 4);
-    h = fnv(h, d.vert_indices, (size_t)d.num_tris * 12);
-    h = fnv(h, d.vertices_uvx, (size_t)d.num_vertices * 16);
-    if (d.normals_uvy) h = fnv(h, d.normals_uvy, (size_t)d.num_vertices * 16);
-    h = fnv(h, d.materials, (size_t)d.num_materials * 112);
-    h = fnv(h, d.transforms, (size_t)d.num_instances * 64);
-    if (d.lights) h = fnv(h, d.lights, (size_t)d.num_lights * 60);
+    h = content_hash(h, d.nodes, (size_t)d.num_nodes * 36);
+    h = content_hash(h, &d.top_level_index, 4);
+    h = content_hash(h, d.vert_indices, (size_t)d.num_tris * 12);
+    h = content_hash(h, d.vertices_uvx, (size_t)d.num_vertices * 16);
+    if (d.normals_uvy) h = content_hash(h, d.normals_uvy, (size_t)d.num_vertices * 16);
+    h = content_hash(h, d.materials, (size_t)d.num_materials * 112);
+    h = content_hash(h, d.transforms, (size_t)d.num_instances * 64);
+    if (d.lights) h = content_hash(h, d.lights, (size_t)d.num_lights * 60);
     if (d.hdr_width > 0) {
         size_t n = (size_t)d.hdr_width * d.hdr_height;
-        h = fnv(h, d.hdr_cols, n * 12);
-        if (d.hdr_marginal) h = fnv(h, d.hdr_marginal, (size_t)d.hdr_height * 8);
-        if (d.hdr_conditional) h = fnv(h, d.hdr_conditional, n * 8);
+        h = content_hash(h, d.hdr_cols, n * 12);
+        if (d.hdr_marginal) h = content_hash(h, d.hdr_marginal, (size_t)d.hdr_height * 8);
+        if (d.hdr_conditional) h = content_hash(h, d.hdr_conditional, n * 8);
     }
     if (d.textures && d.num_textures > 0)
-        h = fnv(h, d.textures, (size_t)d.num_textures * d.tex_width * d.tex_height * 4);
+        h = content_hash(h, d.textures, (size_t)d.num_textures * d.tex_width * d.tex_height * 4);
     if (!d.nodes && d.num_meshes > 0) {  // device BVH build: the mesh and instance tables
-        h = fnv(h, d.mesh_tri_offsets, (size_t)(d.num_meshes + 1) * 8);
-        h = fnv(h, d.instance_meshes, (size_t)d.num_instances * 4);
-        h = fnv(h, d.instance_materials, (size_t)d.num_instances * 4);
+        h = content_hash(h, d.mesh_tri_offsets, (size_t)(d.num_meshes + 1) * 8);
+        h = content_hash(h, d.instance_meshes, (size_t)d.num_instances * 4);
+        h = content_hash(h, d.instance_materials, (size_t)d.num_instances * 4);
     }
     return h;
 }
@@ -386,6 +419,7 @@ int lfh_server_submit(lfh_server* s, const lfh_job_desc* jd, int64_t* job_id) {
     if (o.guiding && o.estimator != 1)
         return sfail(s, LFH_E_INVALID, "lfh_server_submit: guided jobs need options.estimator = 1");
     if (o.estimator != 0 && o.estimator != 1) return sfail(s, LFH_E_INVALID, "lfh_server_submit: bad estimator");
+    if (o.frame_offset < 0) return sfail(s, LFH_E_INVALID, "lfh_server_submit: negative frame offset");
     auto job = std::make_unique<lfh::Job>();
     job->guided = o.guiding != 0;
     job->id = s->next_id++;
@@ -396,6 +430,7 @@ int lfh_server_submit(lfh_server* s, const lfh_job_desc* jd, int64_t* job_id) {
     job->tag = jd->user_tag;
     job->t_submit = s->now();
     job->npix = (int64_t)o.width * o.height;
+    job->frame += o.frame_offset;
     uint64_t h = lfh::hash_desc(*jd->scene);
     auto it = s->scenes.find(h);
     std::shared_ptr<lfh::SceneEntry> e;
@@ -645,15 +680,63 @@ int lfh_server_step(lfh_server* s, int32_t* active) {
         });
         int64_t cap = std::min(lfh::kStepPaths, one_pass + spare);
         cap = std::max(cap, running[0]->npix);
+        // passes per job: what each job could take this step, then the path
+        // budget split between them
+        std::vector<int64_t> want(running.size()), give(running.size(), 0);
+        for (size_t k = 0; k < running.size(); ++k) {
+            lfh::Job* j = running[k];
+            want[k] = std::min<int64_t>((int64_t)j->target - j->spp_done, lfh::kStepMaxPasses);
+            if (j->guided) want[k] = std::min<int64_t>(want[k], lfh::guide_pass_limit(j->scene->grun));
+        }
+        if (LFH_SERVE_FAIR) {
+            // fair shares (water filling): every job is offered an equal part of
+            // the budget; jobs that need less return the rest, which the others
+            // share in the next round; the earliest deadline always gets a pass
+            int64_t left_budget = cap;
+            std::vector<size_t> open;
+            for (size_t k = 0; k < running.size(); ++k) open.push_back(k);
+            while (!open.empty() && left_budget > 0) {
+                const int64_t share = left_budget / (int64_t)open.size();
+                std::vector<size_t> still;
+                int64_t used_round = 0;
+                for (size_t k : open) {
+                    const int64_t fit = share / running[k]->npix;
+                    if (fit >= want[k]) {
+                        give[k] = want[k];
+                        used_round += want[k] * running[k]->npix;
+                    } else {
+                        still.push_back(k);
+                    }
+                }
+                left_budget -= used_round;
+                if (still.size() == open.size()) {
+                    // nobody is satisfied by an equal share: each takes what its share holds
+                    for (size_t k : still) {
+                        give[k] = std::min<int64_t>(want[k], share / running[k]->npix);
+                        left_budget -= give[k] * running[k]->npix;
+                    }
+                    break;
+                }
+                open.swap(still);
+            }
+            if (give[0] == 0 && want[0] > 0) give[0] = 1;   // running[0]: the earliest deadline
+        } else {
+            // earliest deadline first: in that order each job takes what the budget still holds
+            int64_t total0 = 0;
+            for (size_t k = 0; k < running.size(); ++k) {
+                give[k] = std::min<int64_t>(want[k], (cap - total0) / running[k]->npix);
+                if (k == 0) give[k] = std::max<int64_t>(give[k], std::min<int64_t>(want[k], 1));
+                total0 += std::max<int64_t>(give[k], 0) * running[k]->npix;
+            }
+        }
         // groups: the plain jobs together, the guided jobs of each scene together
         std::vector<Group> groups(1);
         std::map<lfh::SceneEntry*, size_t> gi;
         int64_t total = 0;
-        for (lfh::Job* j : running) {
-            const int64_t left = (int64_t)j->target - j->spp_done;
-            int64_t give = std::min<int64_t>(std::min<int64_t>(left, lfh::kStepMaxPasses), (cap - total) / j->npix);
-            if (j->guided) give = std::min<int64_t>(give, lfh::guide_pass_limit(j->scene->grun));
-            if (give <= 0) continue;
+        for (size_t rk = 0; rk < running.size(); ++rk) {
+            lfh::Job* j = running[rk];
+            if (give[rk] <= 0) continue;
+            const int64_t gv = give[rk];
             size_t k = 0;
             if (j->guided) {
                 auto it = gi.find(j->scene.get());
@@ -665,8 +748,8 @@ int lfh_server_step(lfh_server* s, int32_t* active) {
                 k = it->second;
             }
             groups[k].jobs.push_back(j);
-            groups[k].passes.push_back((int)give);
-            total += give * j->npix;
+            groups[k].passes.push_back((int)gv);
+            total += gv * j->npix;
         }
         // job tables of all groups, uploaded together
         std::vector<lfhd::JobDev> tab;

@@ -67,7 +67,7 @@ class Options(C.Structure):
         ("max_depth", C.c_int32), ("enable_rr", C.c_int32), ("rr_depth", C.c_int32),
         ("use_env_map", C.c_int32), ("use_constant_bg", C.c_int32), ("bg_color", C.c_float * 3),
         ("hdr_multiplier", C.c_float), ("kahan", C.c_int32), ("guiding", C.c_int32),
-        ("estimator", C.c_int32), ("reserved", C.c_int32 * 5),
+        ("estimator", C.c_int32), ("frame_offset", C.c_int32), ("reserved", C.c_int32 * 4),
     ]
 
 

@@ -144,3 +144,33 @@ def test_jobs_arriving_while_others_run_equal_rendering_alone():
         m = srv.result(j, o.width, o.height)
         assert np.array_equal(m.view(np.uint32), a.view(np.uint32))
         assert a.sum() > 0
+
+
+def test_frame_offset_gives_independent_samples_and_matches_the_context():
+    """options.frame_offset shifts the random sequence: a served job with
+    offset k equals a context rendering with lfh_set_frame_offset(k) (and with
+    the option given at creation), bitwise, and differs from offset 0."""
+    print("[serve] frame offset", flush=True)
+    sc = scenes.cornell(32, 24)
+    spp, k = 5, 1 << 20
+    imgs = {}
+    for off in (0, k):
+        srv = lfh.Server(max_jobs=2)
+        jid = srv.submit_scene(sc, spp, frame_offset=off)
+        srv.run_until_idle()
+        imgs[off] = srv.result(jid, 32, 24)
+        srv.close()
+    assert not np.array_equal(imgs[0], imgs[k])
+    ctx = lfh.Context.from_scene(sc, tile_width=0, tile_height=0)
+    ctx.options.tile_width, ctx.options.tile_height = 32, 24
+    ctx.set_frame_offset(k)
+    ctx2 = lfh.Context.from_scene(sc, tile_width=0, tile_height=0, frame_offset=k)
+    ctx2.options.tile_width, ctx2.options.tile_height = 32, 24
+    cam = sc.camera()
+    for c in (ctx, ctx2):
+        for _ in range(spp):
+            c.update(cam)
+            c.render()
+    a, b = ctx.output_hdr(), ctx2.output_hdr()
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert np.array_equal(imgs[k].view(np.uint32), a.reshape(imgs[k].shape).view(np.uint32))
