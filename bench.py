@@ -503,12 +503,13 @@ def instanced_leg(stream, cpu=True):
 
 
 # Job sizes: resolutions and sample targets over the configured ranges, small
-# ones the most frequent (about 23M samples per job on average, so 20 jobs/s
-# offer about three quarters of what one GPU renders).
+# ones the most frequent (about 17M samples per job on average, so 20 jobs/s
+# offer about 345 Msamples/s, some 70 % of what one GPU renders on this mix of
+# scenes: a server sized for its load, where queues stay bounded).
 C4_RES = [(256, 256), (512, 512), (640, 360), (960, 540), (1280, 720), (1920, 1080)]
-C4_RES_P = [0.45, 0.25, 0.12, 0.09, 0.05, 0.04]
+C4_RES_P = [0.5, 0.25, 0.1, 0.08, 0.04, 0.03]
 C4_SPP = [16, 32, 64, 128, 256, 512, 1024]
-C4_SPP_P = [0.4, 0.25, 0.15, 0.1, 0.06, 0.03, 0.01]
+C4_SPP_P = [0.45, 0.25, 0.15, 0.08, 0.04, 0.02, 0.01]
 
 
 def _job_stream(n_jobs, rate, seed):
@@ -556,10 +557,12 @@ def _c4_descs(jobs):
     return out
 
 
-def _run_stream(descs, jobs, max_jobs):
+def _run_stream(descs, jobs, max_jobs, policy=None):
     from lavaframe_amd import lfh
     import torch
     srv = lfh.Server(budget_bytes=0, max_jobs=max_jobs)
+    if policy is not None:
+        srv.set_policy(*policy)
     ids = {}
     t0 = time.perf_counter()
     nxt = 0
@@ -632,21 +635,29 @@ def _c4_errors(jobs, results):
     return out
 
 
+C4_JOBS = 80
+
+
 def serving_leg(stream, cpu=True):
-    """Render serving: 60 jobs, Poisson arrivals at 20 jobs/s, from the
+    """Render serving: C4_JOBS jobs, Poisson arrivals at 20 jobs/s, from the
     Cornell, instanced (random seeds) and guiding-room generators at
     256^2 - 1920 x 1080 and 16 - 1024 spp; each scene's BVHs are built on the
     device at admission (inside the latency).  Latency = submit to target spp.
-    Baseline: the same stream with one job admitted at a time (first come,
-    first served), so `cpu` is not used here."""
-    jobs = _job_stream(60, 20.0, seed=0)
+    Batched under the server's default step policy (earliest deadline first)
+    and under fair share; baseline: the same stream with one job admitted at a
+    time (first come, first served), so `cpu` is not used here."""
+    jobs = _job_stream(C4_JOBS, 20.0, seed=0)
     descs = _c4_descs(jobs)
     batched, res_b = _run_stream(descs, jobs, 64)
+    fair, _ = _run_stream(descs, jobs, 64, policy=(1, 4.5e8, 1e9))
     serial, _ = _run_stream(descs, jobs, 1)
-    log(f"serving: batched {batched}, serial {serial}")
+    log(f"serving: batched (deadline order) {batched}")
+    log(f"serving: batched (fair share) {fair}")
+    log(f"serving: serial {serial}")
     return {"serving_c4": {"jobs": len(jobs), "rate_jobs_per_s": 20.0,
                            "samples": int(sum(j["res"][0] * j["res"][1] * j["spp"] for j in jobs)),
-                           "batched": batched, "serial_baseline": serial,
+                           "policy": "earliest deadline first, deadline = submit + samples / 4.5e8 s",
+                           "batched": batched, "batched_fair_share": fair, "serial_baseline": serial,
                            "error_vs_fp64": _c4_errors(jobs, res_b)}}
 
 

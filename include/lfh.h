@@ -373,11 +373,18 @@ void lfh_server_destroy(lfh_server* srv);
 int  lfh_server_submit(lfh_server* srv, const lfh_job_desc* job, int64_t* job_id);
 /* Run one scheduling step: admit queued jobs that fit, render passes of the
  * running jobs in one wavefront (rays of all jobs packed into shared queues;
- * earliest deadline first within a path budget, deadline = submit time +
- * min(samples at a nominal 3e8/s, 0.1 s); the earliest job always advances),
- * retire finished jobs.  *active = number of jobs queued or running
- * afterwards. */
+ * the step's path budget goes to the jobs by the server's policy, see
+ * lfh_server_set_policy; the earliest deadline always advances), retire
+ * finished jobs.  *active = number of jobs queued or running afterwards. */
 int  lfh_server_step(lfh_server* srv, int32_t* active);
+/* Step policy.  A job's deadline is its submit time + min(samples /
+ * nominal_rate, max_offset) seconds.  policy 0: earliest deadline first, each
+ * job in deadline order takes what the step's path budget still holds;
+ * policy 1: fair share, every running job is offered an equal part of the
+ * budget and what a job does not need goes to the others.  Default: policy 0,
+ * nominal_rate 4.5e8 samples/s, no max offset.  No reference counterpart (the
+ * reference renders one scene). */
+int  lfh_server_set_policy(lfh_server* srv, int32_t policy, double nominal_rate, double max_offset);
 int  lfh_server_status(lfh_server* srv, int64_t job_id, lfh_job_status* st);
 /* Result of a finished job: linear HDR (w*h*3) like lfh_get_output_hdr. */
 int  lfh_server_result(lfh_server* srv, int64_t job_id, float* dst, int32_t w, int32_t h);

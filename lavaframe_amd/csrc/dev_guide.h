@@ -113,6 +113,8 @@ LFH_DEV float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16);
 LFH_DEV float vmf_norm(float kappa) {
     return kappa * __builtin_amdgcn_rcpf(kTwoPi * (1.0f - __expf(-2.0f * kappa)));
 }
+// 1 / vmf_norm, for the sampler's choice of lobe.
+LFH_DEV float vmf_inv_norm(float kappa) { return kTwoPi * (1.0f - __expf(-2.0f * kappa)) * __builtin_amdgcn_rcpf(kappa); }
 
 // Processed lobes, kGuideLobeF4 float4 per query: float4 k < 8 is lobe k's
 // (mu, kappa) from outputs 5k..5k+4 = (r, s, a), mu = normalize(r), kappa =
@@ -191,7 +193,7 @@ LFH_DEV float guide_pdf(const float4* __restrict__ lob, V3 d, size_t st = 1) {
     for (int k = 0; k < kGuideLobes; ++k) {
         const float4 l = lob[k * st];
         const float c = l.x * d.x + l.y * d.y + l.z * d.z;
-        p += w[k] * expf(l.w * (c - 1.0f));
+        p += w[k] * __expf(l.w * (c - 1.0f));
     }
     return p;
 }
@@ -215,7 +217,7 @@ LFH_DEV V3 guide_sample(const float4* __restrict__ lob, float u0, float u1, floa
     const float wn[8] = {pa.x, pa.y, pa.z, pa.w, pb.x, pb.y, pb.z, pb.w};
     float pis[8];
 #pragma unroll
-    for (int j = 0; j < kGuideLobes; ++j) pis[j] = wn[j] / vmf_norm(lob[j * st].w);
+    for (int j = 0; j < kGuideLobes; ++j) pis[j] = wn[j] * vmf_inv_norm(lob[j * st].w);
     int k = kGuideLobes - 1;
     float cum = 0.0f;
 #pragma unroll
@@ -228,14 +230,14 @@ LFH_DEV V3 guide_sample(const float4* __restrict__ lob, float u0, float u1, floa
     for (int j = 1; j < kGuideLobes; ++j)
         if (j == k) l = lob[j * st];
     const float kappa = l.w;
-    float w = 1.0f + logf(u1 + (1.0f - u1) * expf(-2.0f * kappa)) / kappa;
+    float w = 1.0f + __logf(u1 + (1.0f - u1) * __expf(-2.0f * kappa)) * __builtin_amdgcn_rcpf(kappa);
     w = fminf(fmaxf(w, -1.0f), 1.0f);
     const float s = sqrtf(fmaxf(0.0f, 1.0f - w * w));
     const V3 mu = v3(l.x, l.y, l.z);
     V3 t, b;
     guide_frame(mu, t, b);
-    const float ph = kTwoPi * u2;
-    const float cp = cosf(ph), sp = sinf(ph);
+    // phi = 2 pi u2: the hardware sin / cos take revolutions
+    const float cp = __builtin_amdgcn_cosf(u2), sp = __builtin_amdgcn_sinf(u2);
     return v3(w * mu.x + s * (cp * t.x + sp * b.x), w * mu.y + s * (cp * t.y + sp * b.y),
               w * mu.z + s * (cp * t.z + sp * b.z));
 }

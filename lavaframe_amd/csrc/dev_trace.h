@@ -142,7 +142,6 @@ struct TraceState {
     int bestPrim, bestInst, bestMat, bestSlot;
     float bu, bv;
     bool occluded;
-    uint32_t leaf;      // postponed BLAS leaf of the current instance (kRefSentinel: none)
 };
 
 LFH_DEV void trace_init(TraceState& s, LaneStack& st, V3 O, V3 D, float tmin, float tmax, uint32_t root) {
@@ -165,7 +164,6 @@ LFH_DEV void trace_init(TraceState& s, LaneStack& st, V3 O, V3 D, float tmin, fl
     s.bu = 0.f;
     s.bv = 0.f;
     s.occluded = false;
-    s.leaf = kRefSentinel;
     st.reset();
 }
 
@@ -264,42 +262,11 @@ LFH_DEV void visit_node(const TraceScene& sc, TraceState& s, LaneStack& st, uint
 }
 
 // Runs up to `budget` loop steps; returns true when the traversal finished.
-#ifndef LFH_POSTPONE_LEAF
-#define LFH_POSTPONE_LEAF 0
-#endif
-#ifndef LFH_SEARCH_MIN
-#define LFH_SEARCH_MIN 0
-#endif
 template <bool ANY, bool STATS>
 LFH_DEV bool trace_run(const TraceScene& sc, TraceState& s, LaneStack& st, float cull, int budget,
                        TraceStats* stats) {
     for (int it = 0; it < budget; ++it) {
         uint32_t cur = s.cur;
-        if (LFH_POSTPONE_LEAF) {
-            // Leaves found during the node walk wait (one per lane) until at
-            // most LFH_SEARCH_MIN lanes of the wave can still walk nodes; then
-            // every waiting lane tests its leaf in the same step (Aila & Laine
-            // 2009, "while-while" with speculative node traversal).  A lane
-            // with a waiting leaf walks on through inner nodes only: a second
-            // leaf, an instance entry or exit needs the waiting leaf's test
-            // first (it is in the current instance's space).
-            const bool pending = s.leaf != kRefSentinel;
-            const bool walks = !pending || (cur != kRefSentinel && !ref_is_leaf(cur) && !ref_is_inst(cur));
-            if (__ballot(pending) != 0 && __popcll(__ballot(walks)) <= LFH_SEARCH_MIN) {
-                if (pending) {
-                    const uint32_t lf = s.leaf;
-                    s.leaf = kRefSentinel;
-                    if (leaf_test<ANY, STATS>(sc, s, lf, stats)) return true;
-                }
-                continue;
-            }
-            if (!walks) continue;
-            if (cur != kRefSentinel && ref_is_leaf(cur)) {
-                s.leaf = cur;
-                s.cur = st.pop_live(s.tbest * cull);
-                continue;
-            }
-        }
         if (cur == kRefSentinel) {
             if (s.inBlas) {
                 s.inBlas = false;

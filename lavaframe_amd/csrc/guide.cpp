@@ -21,11 +21,15 @@ using lfhd::kGuideLobeF4;
 using lfhd::kGuideOut;
 using lfhd::kGuideParams;
 
+// Recorded paths per wavefront (log2) and training samples kept from it (in
+// 65536-sample batches).  Served guiding room, median guided time to RMSE
+// 1e-2 over 10 runs: 2^18 / 4 batches 0.182 s, 2^19 / 8 0.186 s (128 warm-up
+// steps), 2^20 / 16 0.162 s (64 steps; 0.167 s with 128, 0.189 s with 256).
 #ifndef LFH_GUIDE_RECORD_LOG2
-#define LFH_GUIDE_RECORD_LOG2 18
+#define LFH_GUIDE_RECORD_LOG2 20
 #endif
 #ifndef LFH_GUIDE_SAMPLE_BATCHES
-#define LFH_GUIDE_SAMPLE_BATCHES 4
+#define LFH_GUIDE_SAMPLE_BATCHES 16
 #endif
 #ifndef LFH_GUIDE_SERIAL_WARMUP
 #define LFH_GUIDE_SERIAL_WARMUP 0
@@ -36,7 +40,13 @@ constexpr int kBatch = 65536;             // samples per optimisation step
 constexpr int kMaxSamples = LFH_GUIDE_SAMPLE_BATCHES * kBatch;  // samples kept from one wavefront
 constexpr int kRecordedPaths = 1 << LFH_GUIDE_RECORD_LOG2;     // recorded paths per wavefront, at most
 #ifndef LFH_GUIDE_WARM_STEPS
-#define LFH_GUIDE_WARM_STEPS 128   // 64: 0.565, 128: 0.564, 256: 0.604 guided/plain time on the guiding room (later, with faster passes: 64: 0.937 s, 128: 0.950 s guided)
+// Served guiding room, median over 8-10 independent runs of the guided time
+// to RMSE 1e-2 (plain: 0.276 s): warm-up passes / Adam steps / weight cap
+// 4/128/32 0.195 s, 4/64/32 0.174, 8/128/32 0.180, 8/64/32 0.167, 4/128/128
+// 0.161-0.177, 4/64/128 0.173, 8/64/128 0.175, 8/128/128 0.188, 4/128/64
+// 0.191, 8/64/64 0.173, 4/128/1024 0.192; serial warm-up 0.308.  The spread
+// of one median over different run sets is about +-0.015 s.
+#define LFH_GUIDE_WARM_STEPS 64
 #endif
 #ifndef LFH_GUIDE_WARM_REPEATS
 #define LFH_GUIDE_WARM_REPEATS 4
@@ -44,19 +54,19 @@ constexpr int kRecordedPaths = 1 << LFH_GUIDE_RECORD_LOG2;     // recorded paths
 #ifndef LFH_GUIDE_PLAIN_WARMUP
 // > 0: the first passes (one wavefront) sample the BSDF alone and only train
 // the guide; 0: serial warm-up (one pass per wavefront until kWarmSteps).
-// Served guiding room, guided / plain time to RMSE 1e-2 / 5e-3 / 2.5e-3
-// (single runs): serial 1.10 / 0.84 / 0.63; 4 passes 0.63 / 0.74 / 0.86;
-// 8: 0.74 / 0.82 / 0.65; 16: 0.79 / 0.72 / 0.93.
+// Served guiding room, median guided time to RMSE 1e-2 over 8 runs (64 Adam
+// steps, cap 32): serial 0.308 s, 4 passes 0.174 s, 8 passes 0.167 s; with cap
+// 128: 4 passes 0.173 s, 8 passes 0.175 s.
 #define LFH_GUIDE_PLAIN_WARMUP 4
 #endif
 #ifndef LFH_GUIDE_LATE_REPEATS
-#define LFH_GUIDE_LATE_REPEATS 1   // 1: 0.564, 2: 0.638, 4: 0.611 guided/plain time on the guiding room
+#define LFH_GUIDE_LATE_REPEATS 1   // median guided time to 1e-2 over 8 runs: 1 step 0.195 s, 2 steps 0.193 s
 #endif
 constexpr int kWarmSteps = LFH_GUIDE_WARM_STEPS;       // Adam steps taken at the warm-up rate
 constexpr int kLateRepeats = LFH_GUIDE_LATE_REPEATS;   // Adam steps per training pass after the warm-up
 constexpr int kWarmRepeats = LFH_GUIDE_WARM_REPEATS;   // optimisation passes over a pass's samples while warming up
 #ifndef LFH_GUIDE_WEIGHT_CAP
-#define LFH_GUIDE_WEIGHT_CAP 32.0f
+#define LFH_GUIDE_WEIGHT_CAP 128.0f   // the door's rare bright paths need weights far above the mean
 #endif
 constexpr float kWeightCap = LFH_GUIDE_WEIGHT_CAP;   // online training: weights / their mean, capped
 }  // namespace

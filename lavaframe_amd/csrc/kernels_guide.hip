@@ -74,9 +74,15 @@ struct InferQueue {
 // Query inputs of hit-queue entry i: the hit point, the face-forwarded
 // geometric normal in world space, and wo = -dir.  Returns false for misses
 // and light hits (no lobes needed).
-LFH_DEV bool queue_query(const InferQueue& q, uint32_t i, V3& pos, V3& nrm, V3& wo, int& path) {
-    const float4 r0 = q.rayq[(size_t)i * 2], r1 = q.rayq[(size_t)i * 2 + 1];
-    const float4 h0 = q.hitq[(size_t)i * 2], h1 = q.hitq[(size_t)i * 2 + 1];
+// The ray and hit records of entry i (the first loads of a query's chain).
+struct QueueRec {
+    float4 r0, r1, h0, h1;
+};
+LFH_DEV QueueRec load_queue_rec(const InferQueue& q, uint32_t i) {
+    return QueueRec{q.rayq[(size_t)i * 2], q.rayq[(size_t)i * 2 + 1], q.hitq[(size_t)i * 2], q.hitq[(size_t)i * 2 + 1]};
+}
+LFH_DEV bool queue_query(const InferQueue& q, const QueueRec& rec, V3& pos, V3& nrm, V3& wo, int& path) {
+    const float4 r0 = rec.r0, r1 = rec.r1, h0 = rec.h0, h1 = rec.h1;
     path = __float_as_int(r0.w);
     const V3 ro = ld3(r0), rd = ld3(r1);
     wo = -rd;
@@ -159,13 +165,22 @@ __global__ __launch_bounds__(kInferBlock) void k_guide_infer(GuideDev g, const f
     __syncthreads();
     uint16_t* X = sX[wave];
     const uint32_t step = gridDim.x * kInferBlock;
-    for (uint32_t base = (blockIdx.x * kInferWaves + wave) * 64u; base < n; base += step) {
+    const uint32_t first = (blockIdx.x * kInferWaves + wave) * 64u;
+    // hit-queue records one round ahead: their loads are in flight while the
+    // current round runs its tiles.  Lanes past the end load entry n - 1
+    // (unused), so the loads need no branch and the wait before their use
+    // counts only them.
+    QueueRec rec{};
+    if (QUEUE) rec = load_queue_rec(q, min(first + lane, n - 1));
+    for (uint32_t base = first; base < n; base += step) {
         const uint32_t i = base + lane;
+        QueueRec next{};
+        if (QUEUE) next = load_queue_rec(q, min(base + step + lane, n - 1));
         V3 pos = v3(0.f, 0.f, 0.f), nrm = v3(0.f, 0.f, 1.f), wo = v3(0.f, 0.f, 1.f);
         if (i < n) {
             if (QUEUE) {
                 int path;
-                const bool surf = queue_query(q, i, pos, nrm, wo, path);
+                const bool surf = queue_query(q, rec, pos, nrm, wo, path);
                 const GuideHook& gh = q.gh;
                 if (surf && gh.train && (path % gh.stride) == gh.phase) record_query(gh, path, q.depth, pos, nrm, wo);
             } else {
@@ -240,6 +255,7 @@ __global__ __launch_bounds__(kInferBlock) void k_guide_infer(GuideDev g, const f
                 reinterpret_cast<float2*>(dst + (8 + (grp >> 1)) * st)[grp & 1] = make_float2(wk[0], wk[1]);
             }
         }
+        rec = next;
     }
 }
 

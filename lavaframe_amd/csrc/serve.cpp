@@ -55,26 +55,20 @@ constexpr int64_t kAccumBytes = 32;  // sum + compensation, float4 each
 #endif
 constexpr int64_t kStepPaths = (int64_t)1 << LFH_STEP_PATHS_LOG2;   // paths per server step when the budget allows
 constexpr int64_t kStepMaxPasses = 64;    // passes of one job per step, at most
+// Step policy defaults (lfh_server_set_policy changes them per server).
 // A job's deadline: submit time + min(samples / nominal rate, max offset).
-// Small jobs are ordered by size (short ones overtake long ones); a long job is
-// overtaken only by work submitted within the max offset after it, which keeps
-// its latency near that of first-come-first-served.  On a 60-job Poisson
-// stream: offset cap 0.1 s gave p50 0.045 s / p99 0.22 s; no cap 0.043 / 0.25;
-// FIFO order 0.060 / 0.21.
+// Earliest deadline first orders small jobs by size (short ones overtake long
+// ones) while a long job is overtaken only by work whose deadline falls before
+// its own.  Fair share offers every running job an equal part of the step.
+#ifndef LFH_SERVE_POLICY
+#define LFH_SERVE_POLICY 0   // 0: earliest deadline first, 1: fair share
+#endif
 #ifndef LFH_SERVE_RATE
-#define LFH_SERVE_RATE 3.0e8
+#define LFH_SERVE_RATE 4.5e8
 #endif
 #ifndef LFH_SERVE_MAX_OFFSET
-#define LFH_SERVE_MAX_OFFSET 0.1
+#define LFH_SERVE_MAX_OFFSET 1.0e9
 #endif
-// 1: the step's path budget is shared fairly between the running jobs (each
-// offered an equal part, what a job does not need going to the others); 0:
-// earliest deadline first takes what it needs before the next.
-#ifndef LFH_SERVE_FAIR
-#define LFH_SERVE_FAIR 1
-#endif
-constexpr double kDeadlineSecondsPerSample = 1.0 / LFH_SERVE_RATE;
-constexpr double kDeadlineMaxOffset = LFH_SERVE_MAX_OFFSET;
 
 // Content hash of the scene arrays (identical scenes share one resident
 // copy).  Each array: 32-byte blocks into four independent multiply-rotate
@@ -263,6 +257,10 @@ struct lfh_server {
     int uploading = 0;                     // scenes queued or being uploaded
     bool stop = false;
     std::thread worker;
+    // step policy (lfh_server_set_policy)
+    int policy = LFH_SERVE_POLICY;
+    double sec_per_sample = 1.0 / LFH_SERVE_RATE;
+    double max_offset = LFH_SERVE_MAX_OFFSET;
     double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
 };
 
@@ -409,6 +407,15 @@ const char* lfh_server_job_error(lfh_server* s, int64_t id) {
     if (!s) return "";
     auto it = s->jobs.find(id);
     return it == s->jobs.end() ? "unknown job" : it->second->error.c_str();
+}
+
+int lfh_server_set_policy(lfh_server* s, int32_t policy, double nominal_rate, double max_offset) {
+    if (!s || (policy != 0 && policy != 1) || !(nominal_rate > 0.0) || !(max_offset > 0.0))
+        return sfail(s, LFH_E_INVALID, "lfh_server_set_policy: bad arguments");
+    s->policy = policy;
+    s->sec_per_sample = 1.0 / nominal_rate;
+    s->max_offset = max_offset;
+    return LFH_OK;
 }
 
 int lfh_server_submit(lfh_server* s, const lfh_job_desc* jd, int64_t* job_id) {
@@ -656,23 +663,23 @@ int lfh_server_step(lfh_server* s, int32_t* active) {
         j->t_admit = s->now();
         running.push_back(j);
     }
-    // -- passes for this step.  Jobs are served earliest deadline first, a
-    //    job's deadline being its submit time plus its samples at a nominal
-    //    rate, at most kDeadlineMaxOffset: a small job finishes soon after it
-    //    arrives, and a large job is not overtaken by work that arrives after
-    //    its deadline.  In that order each job takes as many of its remaining
-    //    passes as the step's path budget still holds (at most
-    //    kStepMaxPasses; one while its scene's guide is warming up); the
-    //    first always gets one.  Passes of a job are accumulated per pixel in
-    //    frame order, so its image is that of single passes whatever the
-    //    grouping.
+    // -- passes for this step.  Jobs are ordered by deadline, a job's deadline
+    //    being its submit time plus its samples at the policy's nominal rate
+    //    (at most its max offset): a small job finishes soon after it arrives,
+    //    and a large job is not overtaken by work whose deadline is later than
+    //    its own.  Earliest deadline first: in that order each job takes as
+    //    many of its remaining passes as the step's path budget still holds;
+    //    fair share: each is offered an equal part of the budget.  A job takes
+    //    at most kStepMaxPasses (one while its scene's guide is warming up);
+    //    the earliest deadline always gets one.  Passes of a job are
+    //    accumulated per pixel in frame order, so its image is that of single
+    //    passes whatever the grouping.
     if (!running.empty()) {
         int64_t one_pass = 0;
         for (lfh::Job* j : running) one_pass += j->npix;
         const int64_t spare = std::max<int64_t>(0, (s->budget - s->used) / lfh::kPathBytes);
-        auto deadline = [](const lfh::Job* j) {
-            return j->t_submit + std::min(lfh::kDeadlineMaxOffset,
-                                          (double)j->npix * (double)j->target * lfh::kDeadlineSecondsPerSample);
+        auto deadline = [s](const lfh::Job* j) {
+            return j->t_submit + std::min(s->max_offset, (double)j->npix * (double)j->target * s->sec_per_sample);
         };
         std::stable_sort(running.begin(), running.end(), [&](const lfh::Job* a, const lfh::Job* b) {
             const double da = deadline(a), db = deadline(b);
@@ -688,7 +695,7 @@ int lfh_server_step(lfh_server* s, int32_t* active) {
             want[k] = std::min<int64_t>((int64_t)j->target - j->spp_done, lfh::kStepMaxPasses);
             if (j->guided) want[k] = std::min<int64_t>(want[k], lfh::guide_pass_limit(j->scene->grun));
         }
-        if (LFH_SERVE_FAIR) {
+        if (s->policy == 1) {
             // fair shares (water filling): every job is offered an equal part of
             // the budget; jobs that need less return the rest, which the others
             // share in the next round; the earliest deadline always gets a pass

@@ -181,6 +181,7 @@ _SIGS = {
     "lfh_guide_steps": (C.c_int, [_P, i64p]),
     "lfh_get_ray_counts": (C.c_int, [_P, i64p, i64p]),
     "lfh_set_frame_offset": (C.c_int, [_P, C.c_int32]),
+    "lfh_server_set_policy": (C.c_int, [_P, C.c_int32, C.c_double, C.c_double]),
     "lfh_guide_sample": (C.c_int, [_P, f32p, u32p, C.c_int64, f32p, f32p]),
     "lfh_guide_pdf": (C.c_int, [_P, f32p, f32p, C.c_int64, f32p]),
     "lfh_guide_samples_seen": (C.c_int, [_P, i64p]),
@@ -856,6 +857,11 @@ class Server:
         for k, v in overrides.items():
             setattr(o, k, v)
         return self.submit(scene.desc(), o, scene.camera(), target_spp)
+
+    def set_policy(self, policy: int = 0, nominal_rate: float = 4.5e8, max_offset: float = 1e9):
+        """Step policy: 0 earliest deadline first, 1 fair share; deadline =
+        submit + min(samples / nominal_rate, max_offset) seconds."""
+        self._chk(self.lib.lfh_server_set_policy(self.h, int(policy), float(nominal_rate), float(max_offset)))
 
     def step(self) -> int:
         n = C.c_int32()
