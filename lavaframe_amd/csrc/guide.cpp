@@ -232,11 +232,19 @@ namespace lfh {
 // Passes that may share one wavefront: one at a time while a training guide is
 // young (each pass then samples the guide the previous pass trained), any
 // number afterwards.
+#ifndef LFH_GUIDE_RAMP_PASSES
+#define LFH_GUIDE_RAMP_PASSES 0   // > 0: wavefronts right after the warm-up hold at most this many passes
+#endif
+#ifndef LFH_GUIDE_RAMP_STEPS
+#define LFH_GUIDE_RAMP_STEPS 16   // ... until the guide has taken this many steps beyond the warm-up
+#endif
 int guide_pass_limit(const GuideRun& run) {
     const lfh_guide* g = run.guide;
     if (LFH_GUIDE_PLAIN_WARMUP > 0 && g && run.train_every > 0 && run.pass < LFH_GUIDE_PLAIN_WARMUP)
         return LFH_GUIDE_PLAIN_WARMUP - run.pass;   // the warm-up passes share one wavefront
     if (LFH_GUIDE_SERIAL_WARMUP && g && run.train_every > 0 && g->steps < kWarmSteps) return 1;
+    if (LFH_GUIDE_RAMP_PASSES > 0 && g && run.train_every > 0 && g->steps < kWarmSteps + LFH_GUIDE_RAMP_STEPS)
+        return LFH_GUIDE_RAMP_PASSES;
     return std::numeric_limits<int>::max();
 }
 
