@@ -174,3 +174,32 @@ def test_frame_offset_gives_independent_samples_and_matches_the_context():
     a, b = ctx.output_hdr(), ctx2.output_hdr()
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
     assert np.array_equal(imgs[k].view(np.uint32), a.reshape(imgs[k].shape).view(np.uint32))
+
+
+def test_step_policy_changes_scheduling_not_images():
+    """Earliest deadline first and fair share group passes differently, but a
+    job's passes are accumulated per pixel in frame order: the images are
+    bitwise the same under both policies, and the policy call rejects bad
+    arguments."""
+    print("[serve] step policies", flush=True)
+    jobs = _jobs()
+    out = []
+    for policy in (0, 1):
+        srv = lfh.Server(max_jobs=8)
+        srv.set_policy(policy, 4.5e8, 1e9)
+        ids = [srv.submit_scene(sc, spp) for sc, spp in jobs]
+        srv.run_until_idle()
+        imgs = []
+        for (sc, spp), jid in zip(jobs, ids):
+            st = srv.status(jid)
+            assert st.state == 2 and st.spp_done == spp
+            o = sc.options()
+            imgs.append(srv.result(jid, o.width, o.height))
+        with pytest.raises(lfh.LfhError):
+            srv.set_policy(2, 4.5e8, 1e9)
+        with pytest.raises(lfh.LfhError):
+            srv.set_policy(0, 0.0, 1e9)
+        srv.close()
+        out.append(imgs)
+    for a, b in zip(*out):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
