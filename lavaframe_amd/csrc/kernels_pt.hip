@@ -392,7 +392,11 @@ LFH_DEV uint32_t bin_by_class(int cls, uint32_t* lds) {
 
 template <bool MULTI>
 #ifndef LFH_SHADE_WAVES
-#define LFH_SHADE_WAVES 1   // k_shade register budget in waves per SIMD; 1 (165 VGPRs, 3 waves): C0 400 / C2 249 Msamples/s, 4 (156 B/lane spilled): 414 / 249, 5: 409 / 248
+// k_shade register budget in waves per SIMD.  1 (153-158 VGPRs, 3 waves): C0 1207 /
+// C2 391 Msamples/s, C3 1.463 / 2.134 ms per plain / guided pass; 4 (128 VGPRs,
+// 100-140 B/lane spilled): 1208 / 384, 1.506 / 2.216; 5 (96 VGPRs, 244-276 B):
+// 1037 / 363, 1.607 / 2.358.  The spills cost more than the occupancy gains.
+#define LFH_SHADE_WAVES 1
 #endif
 __global__ __launch_bounds__(256, LFH_SHADE_WAVES) void k_shade(ShadeScene sc1, const JobDev* __restrict__ jobs, int depth,
                                                const float4* __restrict__ rayq,
