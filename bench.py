@@ -186,6 +186,10 @@ def main():
     issue = read_issue(ROOT / "profiles" / "traversal_c1_sq.txt")
     if issue:
         roofline["issue"] = issue
+        if issue.get("valu_busy_frac", 0.0) > 0.8:
+            # the kernel is bound by VALU issue, not by bytes: HBM frac is the counter fraction
+            roofline["limiter"] = (f"VALU issue: SIMD VALU {issue['valu_busy_frac']:.0%} busy "
+                                   "(SQ_ACTIVE_INST_VALU, profiles/traversal_c1_sq.txt)")
     if not args.profile_only:
         cp = copy_peak(stream)
         roofline["copy_peak"] = round(cp, 1)
