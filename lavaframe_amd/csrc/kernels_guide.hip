@@ -326,6 +326,7 @@ __global__ __launch_bounds__(kTrainBlock) void k_guide_train(GuideDev g, TrainAr
     uint16_t* sG1 = reinterpret_cast<uint16_t*>(sOG);
     __shared__ float sD[kTrainRows][4];
     __shared__ float sLoss[kTrainRows];
+    __shared__ float sDb[4][64];   // quarter sums of the bias gradient
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int lr = lane & 15, lk = (lane >> 4) * 8, lq = (lane >> 4) * 4;
     const int N = batch_size(ta);
@@ -391,58 +392,70 @@ __global__ __launch_bounds__(kTrainBlock) void k_guide_train(GuideDev g, TrainAr
     }
     __syncthreads();
 
-    // 3. d loss / d outputs (one thread per sample), BF16 into sG0 (cols 40..63 zero)
+    // 3. d loss / d outputs, BF16 into sG0 (cols 40..63 zero).  Four threads per
+    // sample, in one wave: sample 16 wave + lane % 16, lane group g = lane / 16
+    // takes lobes 2g and 2g+1; the softmax and log-sum-exp over the 8 lobes
+    // reduce over the 4 groups with shuffles (every group ends with the same
+    // sums: the two-step xor tree adds the same pairs in every lane).
     const float invN = 1.0f / (float)N;
-    if (tid < kTrainRows) {
-        const int s = tid;
+    {
+        const int s = wave * 16 + lr, grp = lane >> 4;
         const float* o = sO + s * kOutPitch;
         uint16_t* grow = sG0 + s * kPitch;
         const float w = sD[s][3];
-        float lossv = 0.0f;
-        for (int c = kGuideOut; c < 64; ++c) grow[c] = 0;
-        if (w != 0.0f) {
-            const V3 d = v3(sD[s][0], sD[s][1], sD[s][2]);
-            float amax = -3.0e38f;
-            for (int k = 0; k < kGuideLobes; ++k) amax = fmaxf(amax, o[5 * k + 4]);
-            float pi[kGuideLobes], lw[kGuideLobes], kap[kGuideLobes], cs[kGuideLobes], rn[kGuideLobes];
-            V3 mu[kGuideLobes];
-            float psum = 0.0f;
-            for (int k = 0; k < kGuideLobes; ++k) {
-                pi[k] = expf(o[5 * k + 4] - amax);
-                psum += pi[k];
-            }
-            float lmax = -3.0e38f;
-            for (int k = 0; k < kGuideLobes; ++k) {
-                pi[k] /= psum;
-                const V3 r = v3(o[5 * k], o[5 * k + 1], o[5 * k + 2]);
-                rn[k] = fmaxf(sqrtf(dot(r, r)), 1e-12f);
-                mu[k] = r * (1.0f / rn[k]);
-                kap[k] = expf(fminf(fmaxf(o[5 * k + 3], kLogKMin), kLogKMax));
-                cs[k] = dot(mu[k], d);
-                lw[k] = logf(pi[k]) + logf(vmf_norm(kap[k])) + kap[k] * (cs[k] - 1.0f);
-                lmax = fmaxf(lmax, lw[k]);
-            }
-            float gs = 0.0f;
-            for (int k = 0; k < kGuideLobes; ++k) gs += expf(lw[k] - lmax);
-            lossv = -w * (lmax + logf(gs)) * invN;
-            const float sc = w * invN;
-            for (int k = 0; k < kGuideLobes; ++k) {
-                const float gam = expf(lw[k] - lmax) / gs;
-                const float e2 = expf(-2.0f * kap[k]);
-                const float dlogv = 1.0f / kap[k] - 2.0f * e2 / (-expm1f(-2.0f * kap[k])) + (cs[k] - 1.0f);
+        for (int c = kGuideOut + grp * 6; c < kGuideOut + grp * 6 + 6; ++c) grow[c] = 0;
+        const V3 d = v3(sD[s][0], sD[s][1], sD[s][2]);
+        const int k0 = 2 * grp;
+        float amax = fmaxf(o[5 * k0 + 4], o[5 * k0 + 9]);
+        amax = fmaxf(amax, __shfl_xor(amax, 16));
+        amax = fmaxf(amax, __shfl_xor(amax, 32));
+        float pi[2], lw[2], kap[2], cs[2], rn[2];
+        V3 mu[2];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) pi[j] = expf(o[5 * (k0 + j) + 4] - amax);
+        float psum = pi[0] + pi[1];
+        psum += __shfl_xor(psum, 16);
+        psum += __shfl_xor(psum, 32);
+        float lmax = -3.0e38f;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int k = k0 + j;
+            pi[j] /= psum;
+            const V3 r = v3(o[5 * k], o[5 * k + 1], o[5 * k + 2]);
+            rn[j] = fmaxf(sqrtf(dot(r, r)), 1e-12f);
+            mu[j] = r * (1.0f / rn[j]);
+            kap[j] = expf(fminf(fmaxf(o[5 * k + 3], kLogKMin), kLogKMax));
+            cs[j] = dot(mu[j], d);
+            lw[j] = logf(pi[j]) + logf(vmf_norm(kap[j])) + kap[j] * (cs[j] - 1.0f);
+            lmax = fmaxf(lmax, lw[j]);
+        }
+        lmax = fmaxf(lmax, __shfl_xor(lmax, 16));
+        lmax = fmaxf(lmax, __shfl_xor(lmax, 32));
+        float gs = expf(lw[0] - lmax) + expf(lw[1] - lmax);
+        gs += __shfl_xor(gs, 16);
+        gs += __shfl_xor(gs, 32);
+        const float sc = w * invN;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int k = k0 + j;
+            if (w != 0.0f) {
+                const float gam = expf(lw[j] - lmax) / gs;
+                const float e2 = expf(-2.0f * kap[j]);
+                const float dlogv = 1.0f / kap[j] - 2.0f * e2 / (-expm1f(-2.0f * kap[j])) + (cs[j] - 1.0f);
                 const float sraw = o[5 * k + 3];
                 const bool inside = sraw > kLogKMin && sraw < kLogKMax;
-                const float gr = -sc * gam * kap[k] / rn[k];
-                grow[5 * k + 0] = to_bf16(gr * (d.x - cs[k] * mu[k].x));
-                grow[5 * k + 1] = to_bf16(gr * (d.y - cs[k] * mu[k].y));
-                grow[5 * k + 2] = to_bf16(gr * (d.z - cs[k] * mu[k].z));
-                grow[5 * k + 3] = to_bf16(inside ? -sc * gam * kap[k] * dlogv : 0.0f);
-                grow[5 * k + 4] = to_bf16(sc * (pi[k] - gam));
+                const float gr = -sc * gam * kap[j] / rn[j];
+                grow[5 * k + 0] = to_bf16(gr * (d.x - cs[j] * mu[j].x));
+                grow[5 * k + 1] = to_bf16(gr * (d.y - cs[j] * mu[j].y));
+                grow[5 * k + 2] = to_bf16(gr * (d.z - cs[j] * mu[j].z));
+                grow[5 * k + 3] = to_bf16(inside ? -sc * gam * kap[j] * dlogv : 0.0f);
+                grow[5 * k + 4] = to_bf16(sc * (pi[j] - gam));
+            } else {
+#pragma unroll
+                for (int c = 0; c < 5; ++c) grow[5 * k + c] = 0;
             }
-        } else {
-            for (int c = 0; c < kGuideOut; ++c) grow[c] = 0;
         }
-        sLoss[s] = lossv;
+        if (grp == 0) sLoss[s] = w != 0.0f ? -w * (lmax + logf(gs)) * invN : 0.0f;
     }
     __syncthreads();
     if (tid < 64) {
@@ -490,12 +503,17 @@ __global__ __launch_bounds__(kTrainBlock) void k_guide_train(GuideDev g, TrainAr
                 }
             }
         }
-        // db[o] = sum_s dY[s][o]
-        if (tid < nout) {
+        // db[o] = sum_s dY[s][o]: four quarter sums of 16 rows per output,
+        // then added in quarter order
+        {
+            const int o = tid & 63, q = tid >> 6;
             float sum = 0.0f;
-            for (int s = 0; s < kTrainRows; ++s) sum += from_bf16(dY[s * kPitch + tid]);
-            part[off + 64 * nout + tid] = sum;
+#pragma unroll
+            for (int s = 0; s < kTrainRows / 4; ++s) sum += from_bf16(dY[(q * (kTrainRows / 4) + s) * kPitch + o]);
+            sDb[q][o] = sum;
         }
+        __syncthreads();
+        if (tid < nout) part[off + 64 * nout + tid] = ((sDb[0][tid] + sDb[1][tid]) + sDb[2][tid]) + sDb[3][tid];
         if (layer > 0) {
             // dX[s][i] = (sum_o dY[s][o] W[i][o]) * (X[s][i] > 0), rows of this wave
             const uint16_t* WB = g.wb + layer * 4096;
