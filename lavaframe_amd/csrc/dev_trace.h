@@ -142,7 +142,48 @@ struct TraceState {
     int bestPrim, bestInst, bestMat, bestSlot;
     float bu, bv;
     bool occluded;
+#if LFH_SLAB_OCTANT
+#if LFH_OCTANT_PACKED
+    uint32_t oct;   // near-plane byte offsets x | y << 8 | z << 16, bit 24: inv has an infinite component
+#else
+    uint32_t nearx, neary, nearz;   // byte offsets of the near-side box planes in a node (ray octant)
+    bool inf_inv;                   // some component of inv is infinite (axis-parallel ray)
+#endif
+#endif
 };
+
+#if LFH_SLAB_OCTANT
+// The near-side plane of each axis follows from the sign of inv: min at
+// bytes 0/16/32 of a node, max at 48/64/80.  For finite inv the near plane's
+// distance equals the per-axis min of the two (t = (b - o) inv is monotonic
+// in b), bit for bit; an infinite component (d = +-0) can make one of them
+// 0 * inf = NaN, which the min/max form resolves differently, so such rays
+// take the min/max form.
+LFH_DEV void set_octant(TraceState& s) {
+    const uint32_t nx = signbit(s.inv.x) ? 48u : 0u, ny = signbit(s.inv.y) ? 64u : 16u,
+                   nz = signbit(s.inv.z) ? 80u : 32u;
+    const bool inf = !(isfinite(s.inv.x) && isfinite(s.inv.y) && isfinite(s.inv.z));
+#if LFH_OCTANT_PACKED
+    s.oct = nx | (ny << 8) | (nz << 16) | (inf ? 1u << 24 : 0u);
+#else
+    s.nearx = nx;
+    s.neary = ny;
+    s.nearz = nz;
+    s.inf_inv = inf;
+#endif
+}
+#if LFH_OCTANT_PACKED
+LFH_DEV bool octant_inf(const TraceState& s) { return (s.oct >> 24) != 0; }
+LFH_DEV uint32_t near_x(const TraceState& s) { return s.oct & 0xFFu; }
+LFH_DEV uint32_t near_y(const TraceState& s) { return (s.oct >> 8) & 0xFFu; }
+LFH_DEV uint32_t near_z(const TraceState& s) { return (s.oct >> 16) & 0xFFu; }
+#else
+LFH_DEV bool octant_inf(const TraceState& s) { return s.inf_inv; }
+LFH_DEV uint32_t near_x(const TraceState& s) { return s.nearx; }
+LFH_DEV uint32_t near_y(const TraceState& s) { return s.neary; }
+LFH_DEV uint32_t near_z(const TraceState& s) { return s.nearz; }
+#endif
+#endif
 
 LFH_DEV void trace_init(TraceState& s, LaneStack& st, V3 O, V3 D, float tmin, float tmax, uint32_t root) {
     s.O = O;
@@ -164,6 +205,9 @@ LFH_DEV void trace_init(TraceState& s, LaneStack& st, V3 O, V3 D, float tmin, fl
     s.bu = 0.f;
     s.bv = 0.f;
     s.occluded = false;
+#if LFH_SLAB_OCTANT
+    set_octant(s);
+#endif
     st.reset();
 }
 
@@ -212,6 +256,9 @@ LFH_DEV void enter_instance(const TraceScene& sc, TraceState& s, LaneStack& st, 
     s.o = xform_inv(r0, r1, r2, s.O, 1.0f);
     s.d = xform_inv(r0, r1, r2, s.D, 0.0f);
     s.inv = rcp3(s.d);
+#if LFH_SLAB_OCTANT
+    set_octant(s);
+#endif
     st.push(kRefSentinel, -kInfinity);
     s.inBlas = true;
     s.curInst = (int)inst;
@@ -224,17 +271,40 @@ LFH_DEV void enter_instance(const TraceScene& sc, TraceState& s, LaneStack& st, 
 template <bool STATS>
 LFH_DEV void visit_node(const TraceScene& sc, TraceState& s, LaneStack& st, uint32_t cur, float cull,
                         TraceStats* stats) {
-    const float4* np = sc.nodes + (size_t)cur * 8;
-    const float4 mnx = np[0], mny = np[1], mnz = np[2], mxx = np[3], mxy = np[4], mxz = np[5];
-    const float4 rf = np[6];
     if (STATS) stats->nint++;
     const float lim = s.tbest * cull;
     float h[4], e[4];
-    uint32_t r[4] = {as_u(rf.x), as_u(rf.y), as_u(rf.z), as_u(rf.w)};
-    h[0] = slab(v3(mnx.x, mny.x, mnz.x), v3(mxx.x, mxy.x, mxz.x), s.o, s.inv, &e[0]);
-    h[1] = slab(v3(mnx.y, mny.y, mnz.y), v3(mxx.y, mxy.y, mxz.y), s.o, s.inv, &e[1]);
-    h[2] = slab(v3(mnx.z, mny.z, mnz.z), v3(mxx.z, mxy.z, mxz.z), s.o, s.inv, &e[2]);
-    h[3] = slab(v3(mnx.w, mny.w, mnz.w), v3(mxx.w, mxy.w, mxz.w), s.o, s.inv, &e[3]);
+    uint32_t r[4];
+#if LFH_SLAB_OCTANT
+    if (__ballot(octant_inf(s)) == 0) {   // wave-uniform
+        // near and far planes read by per-lane offsets: the per-axis min/max drop out
+        const char* nb = reinterpret_cast<const char*>(sc.nodes);
+        const uint32_t off = cur * 128u;
+        const uint32_t ox = near_x(s), oy = near_y(s), oz = near_z(s);
+        const float4 nx = *reinterpret_cast<const float4*>(nb + (off + ox));
+        const float4 ny = *reinterpret_cast<const float4*>(nb + (off + oy));
+        const float4 nz = *reinterpret_cast<const float4*>(nb + (off + oz));
+        const float4 fx = *reinterpret_cast<const float4*>(nb + (off + (48u - ox)));
+        const float4 fy = *reinterpret_cast<const float4*>(nb + (off + (80u - oy)));
+        const float4 fz = *reinterpret_cast<const float4*>(nb + (off + (112u - oz)));
+        const float4 rf = *reinterpret_cast<const float4*>(nb + (off + 96u));
+        r[0] = as_u(rf.x); r[1] = as_u(rf.y); r[2] = as_u(rf.z); r[3] = as_u(rf.w);
+        h[0] = slab_octant(v3(nx.x, ny.x, nz.x), v3(fx.x, fy.x, fz.x), s.o, s.inv, &e[0]);
+        h[1] = slab_octant(v3(nx.y, ny.y, nz.y), v3(fx.y, fy.y, fz.y), s.o, s.inv, &e[1]);
+        h[2] = slab_octant(v3(nx.z, ny.z, nz.z), v3(fx.z, fy.z, fz.z), s.o, s.inv, &e[2]);
+        h[3] = slab_octant(v3(nx.w, ny.w, nz.w), v3(fx.w, fy.w, fz.w), s.o, s.inv, &e[3]);
+    } else
+#endif
+    {
+        const float4* np = sc.nodes + (size_t)cur * 8;
+        const float4 mnx = np[0], mny = np[1], mnz = np[2], mxx = np[3], mxy = np[4], mxz = np[5];
+        const float4 rf = np[6];
+        r[0] = as_u(rf.x); r[1] = as_u(rf.y); r[2] = as_u(rf.z); r[3] = as_u(rf.w);
+        h[0] = slab(v3(mnx.x, mny.x, mnz.x), v3(mxx.x, mxy.x, mxz.x), s.o, s.inv, &e[0]);
+        h[1] = slab(v3(mnx.y, mny.y, mnz.y), v3(mxx.y, mxy.y, mxz.y), s.o, s.inv, &e[1]);
+        h[2] = slab(v3(mnx.z, mny.z, mnz.z), v3(mxx.z, mxy.z, mxz.z), s.o, s.inv, &e[2]);
+        h[3] = slab(v3(mnx.w, mny.w, mnz.w), v3(mxx.w, mxy.w, mxz.w), s.o, s.inv, &e[3]);
+    }
     int nv = 0;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -273,6 +343,9 @@ LFH_DEV bool trace_run(const TraceScene& sc, TraceState& s, LaneStack& st, float
                 s.o = s.O;
                 s.d = s.D;
                 s.inv = s.invD;
+#if LFH_SLAB_OCTANT
+                set_octant(s);
+#endif
                 s.cur = st.pop_live(s.tbest * cull);
                 continue;
             }

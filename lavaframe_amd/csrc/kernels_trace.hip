@@ -15,7 +15,7 @@ __global__ void k_zero_fetch(unsigned long long* work) {
 }
 
 template <bool ANY, bool STATS>
-__global__ __launch_bounds__(kTraceBlock, LFH_TRACE_WAVES) void k_trace(TraceScene sc, const float4* __restrict__ rays, int64_t n,
+__global__ __launch_bounds__(kTraceBlock, ANY ? LFH_ANYHIT_WAVES : LFH_TRACE_WAVES) void k_trace(TraceScene sc, const float4* __restrict__ rays, int64_t n,
                                                        int32_t* __restrict__ hits, uint8_t* __restrict__ occl,
                                                        int32_t* __restrict__ counts, uint32_t* __restrict__ spill,
                                                        unsigned long long* __restrict__ work, float cull,
