@@ -233,7 +233,7 @@ LFH_DEV bool light_any(const ShadeScene& sc, V3 o, V3 d, float maxDist) {
 
 template <bool SHADOW, bool MULTI>
 #ifndef LFH_QUEUE_WAVES
-#define LFH_QUEUE_WAVES 6   // queue traversal register budget, waves per SIMD; 6 (80 VGPRs, up to 36 B/lane spilled): C0 1216 / C2 388 Msamples/s, 1 (88 VGPRs, 5 waves): 1226 / 368 (2^24-path wavefronts)
+#define LFH_QUEUE_WAVES 6   // queue traversal register budget, waves per SIMD; with the octant slab 6: C0 1206 / C2 396 Msamples/s, 5: 1223 / 379 (C3 pass cost equal)
 #endif
 __global__ __launch_bounds__(kTraceBlock, LFH_QUEUE_WAVES) void k_trace_queue(ShadeScene sc, const JobDev* __restrict__ jobs,
                                                              const float4* __restrict__ q,
