@@ -27,7 +27,10 @@ constexpr int kGuideLayers = 4;
 constexpr int kGuideParams = 3 * (64 * 64 + 64) + 64 * 40 + 40;  // 15080
 constexpr float kLogKMin = -3.0f;
 constexpr float kLogKMax = 7.0f;
-constexpr float kGuideFrac = 0.5f;   // probability of drawing from the guide
+#ifndef LFH_GUIDE_FRAC
+#define LFH_GUIDE_FRAC 0.5f
+#endif
+constexpr float kGuideFrac = LFH_GUIDE_FRAC;   // probability of drawing from the guide
 constexpr int kGuideLobeF4 = 10;     // processed lobes per query: 8 x (mu, kappa), 2 x (pi * vmf_norm)[4]
 
 LFH_HD int guide_layer_out(int l) { return l == 3 ? kGuideOut : kGuideHid; }

@@ -92,23 +92,34 @@ static int occupancy_blocks(const void* fn) {
     return nb * prop.multiProcessorCount;
 }
 
+// Resident blocks of each variant on the whole device: closest hit, any hit, closest hit with counts.
+// The variants are register-budgeted for different waves per SIMD, so each launches its own
+// resident count; blocks beyond it would only start once the work queue is empty.
+struct VariantBlocks {
+    int n[3];
+};
+
+static const int* variant_blocks() {
+    static const VariantBlocks blocks = {{occupancy_blocks((const void*)k_trace<false, false>),
+                                          occupancy_blocks((const void*)k_trace<true, false>),
+                                          occupancy_blocks((const void*)k_trace<false, true>)}};
+    return blocks.n;
+}
+
 int lfh_trace_grid_blocks() {
-    static int blocks = 0;
-    if (blocks == 0) {
-        int a = occupancy_blocks((const void*)k_trace<false, false>);
-        int b = occupancy_blocks((const void*)k_trace<true, false>);
-        int c = occupancy_blocks((const void*)k_trace<false, true>);
-        blocks = a > b ? a : b;
-        blocks = blocks > c ? blocks : c;
-    }
-    return blocks;
+    const int* v = variant_blocks();
+    int blocks = v[0] > v[1] ? v[0] : v[1];
+    return blocks > v[2] ? blocks : v[2];
 }
 
 int lfh_trace_threads_per_block() { return kTraceBlock; }
 uint32_t lfh_trace_spill_stride() { return kSpillStride; }
 
 hipError_t lfh_launch_trace(const TraceLaunch& L, hipStream_t stream) {
-    const int blocks = L.grid_blocks;
+    // L.grid_blocks sizes the spill buffer; each variant launches at most its own resident count
+    const int* v = variant_blocks();
+    const int which = L.any ? 1 : (L.counts ? 2 : 0);
+    const int blocks = v[which] < L.grid_blocks ? v[which] : L.grid_blocks;
     hipLaunchKernelGGL(k_zero_fetch, dim3(1), dim3(64), 0, stream, L.work);
     if (L.any) {
         hipLaunchKernelGGL((k_trace<true, false>), dim3(blocks), dim3(kTraceBlock), 0, stream, L.scene, L.rays, L.n,
