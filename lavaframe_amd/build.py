@@ -28,12 +28,17 @@ COMMON = ["-std=c++17", "-O3", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "
           f"-I{PKG.parent / 'include'}"]
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-munsafe-fp-atomics"]
+# Per-source device flags.  The guide kernels keep MFMA accumulators in
+# architectural VGPRs: the packing of one layer's results into the next
+# layer's operand then reads them directly, without a v_accvgpr_read per
+# register (58 per 16-query tile of the inference kernel).
+DEVICE_EXTRA = {"kernels_guide.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]}
 
 
 def _compile(src: Path) -> Path:
     obj = OBJ_DIR / (src.name + ".o")
     if src.suffix == ".hip":
-        cmd = [HIPCC, *COMMON, *DEVICE, "-x", "hip", "-c", str(src), "-o", str(obj)]
+        cmd = [HIPCC, *COMMON, *DEVICE, *DEVICE_EXTRA.get(src.name, []), "-x", "hip", "-c", str(src), "-o", str(obj)]
     else:
         cmd = [HOSTCXX, *COMMON, *HOST, "-c", str(src), "-o", str(obj)]
     r = subprocess.run(cmd, capture_output=True, text=True)

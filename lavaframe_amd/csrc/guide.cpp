@@ -69,6 +69,18 @@ constexpr int kWarmRepeats = LFH_GUIDE_WARM_REPEATS;   // optimisation passes ov
 #define LFH_GUIDE_WEIGHT_CAP 128.0f   // the door's rare bright paths need weights far above the mean
 #endif
 constexpr float kWeightCap = LFH_GUIDE_WEIGHT_CAP;   // online training: weights / their mean, capped
+#ifndef LFH_GUIDE_LATE_EVERY
+// Served guiding room, median over 6 runs of the guided time to RMSE 1e-2 / 5e-3:
+// every pass 0.156 / 0.606 s; every 2nd pass after 128 steps 0.156 / 0.586 s
+// (2.106 ms per pass against 2.12); every 2nd or 4th pass after 64 steps 0.167 /
+// 0.657 s and 0.187 / 0.624 s.
+#define LFH_GUIDE_LATE_EVERY 2
+#endif
+#ifndef LFH_GUIDE_LATE_AFTER
+#define LFH_GUIDE_LATE_AFTER 128
+#endif
+constexpr int kLateEvery = LFH_GUIDE_LATE_EVERY;   // once kLateAfter steps are taken, one training pass in kLateEvery
+constexpr int kLateAfter = LFH_GUIDE_LATE_AFTER;
 }  // namespace
 
 struct lfh_guide {
@@ -254,8 +266,10 @@ namespace {
 // Training passes among the wavefront's passes run.pass .. run.pass + batch - 1.
 int training_passes(const GuideRun& run, int max_depth, int batch) {
     if (run.train_every <= 0 || max_depth <= 0) return 0;
+    const bool late = run.guide && run.guide->steps >= kLateAfter;
+    const int every = run.train_every * (late ? kLateEvery : 1);
     int n = 0;
-    for (int i = 0; i < batch; ++i) n += ((run.pass + i) % run.train_every) == 0;
+    for (int i = 0; i < batch; ++i) n += ((run.pass + i) % every) == 0;
     return n;
 }
 

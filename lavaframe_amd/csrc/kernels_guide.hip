@@ -259,6 +259,17 @@ __global__ __launch_bounds__(kInferBlock) void k_guide_infer(GuideDev g, const f
     }
 }
 
+// Passes that draw no direction from the guide (the BSDF-only warm-up) need no
+// lobes, only the training records of the hit queue's recorded paths.
+__global__ __launch_bounds__(256) void k_guide_record(InferQueue q) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= *q.count) return;
+    V3 pos, nrm, wo;
+    int path;
+    if (queue_query(q, load_queue_rec(q, i), pos, nrm, wo, path) && (path % q.gh.stride) == q.gh.phase)
+        record_query(q.gh, path, q.depth, pos, nrm, wo);
+}
+
 // ---- mixture sampling / pdf on processed lobes (checks of the device sampler) ----
 __global__ void k_guide_sample(const float4* __restrict__ lobes, const uint4* __restrict__ seeds, int n,
                                float* __restrict__ dirs, float* __restrict__ pdf) {
@@ -755,6 +766,10 @@ hipError_t guide_infer_queue(const GuideDev& g, const ShadeScene& sc, const floa
                              hipStream_t s) {
     if (maxq <= 0) return hipSuccess;
     InferQueue q{sc, rayq, hitq, count, gh, depth};
+    if (!gh.sample) {
+        if (gh.train) hipLaunchKernelGGL(k_guide_record, dim3(nblk(maxq, 256)), dim3(256), 0, s, q);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(k_guide_infer<true>, dim3(infer_blocks(maxq)), dim3(kInferBlock), 0, s, g,
                        (const float*)nullptr, 0, q, (float*)nullptr, lobes);
     return hipGetLastError();
