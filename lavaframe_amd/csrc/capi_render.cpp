@@ -99,17 +99,29 @@ int render_init(lfh_ctx* c) {
         c->pt_grid = pt_trace_grid_blocks();
         size_t need = (size_t)c->pt_grid * lfh_trace_threads_per_block() * lfh_trace_spill_stride() * 4;
         if (need > c->spill.bytes) e = c->spill.alloc(need);
+        if (e == hipSuccess && shadow_lane_enabled()) e = c->lane.init(need);
     }
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return rhip(c, e, "render_init");
     return LFH_OK;
 }
 
-void render_release(lfh_ctx* c) { c->bgraph.reset(); }
+void render_release(lfh_ctx* c) {
+    c->bgraph.reset();
+    c->lane.release();
+}
 
 bool bounce_graphs_enabled() {
     static const bool on = [] {
         const char* e = std::getenv("LFH_BOUNCE_GRAPHS");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
+bool shadow_lane_enabled() {
+    static const bool on = [] {
+        const char* e = std::getenv("LFH_SHADOW_LANE");
         return !(e && e[0] == '0');
     }();
     return on;
@@ -181,6 +193,8 @@ int render_tile(lfh_ctx* c, int batch) {
     GuideHook gh{};
     lfhd::GuideDev gd{};
     if (guide_pass_begin(c, batch, &gh, &gd) < 0) return rfail(c, LFH_E_HIP, "render: guide buffers");
+    const ShadowLane lane = c->lane.lane();
+    const ShadowLane* lp = c->lane.ready() ? &lane : nullptr;
     auto chain = [&]() -> hipError_t {
         for (int depth = 0; depth < c->opt.max_depth; ++depth) {
             const int cur = depth & 1;
@@ -188,15 +202,16 @@ int render_tile(lfh_ctx* c, int batch) {
             hipError_t e = pt_bounce(sc, depth, pb.cap, q[cur], cnt + cur, q[cur ^ 1], cnt + (cur ^ 1),
                                      pb.hq.as<float4>(), pb.sq.as<float4>(), cnt + 2, lfh::view(pb),
                                      c->spill.as<uint32_t>(), work, c->rflag.as<int>(), c->cull, c->pt_grid, gh, &gd,
-                                     c->stream);
+                                     c->stream, lp);
             if (e != hipSuccess) return e;
         }
-        return hipSuccess;
+        return lp && c->opt.max_depth > 0 ? pt_bounce_join(*lp, c->stream) : hipSuccess;
     };
     if (!gh.enabled && bounce_graphs_enabled()) {
         // the guide hook changes every pass; unguided chains are replayed
-        const std::vector<uintptr_t> key = {(uintptr_t)pb.rad.p, (uintptr_t)pb.cap, (uintptr_t)c->opt.max_depth,
-                                            (uintptr_t)c->stream, (uintptr_t)c->spill.p, (uintptr_t)c->scene_version};
+        const std::vector<uintptr_t> key = {(uintptr_t)pb.rad.p,  (uintptr_t)pb.cap,     (uintptr_t)c->opt.max_depth,
+                                            (uintptr_t)c->stream, (uintptr_t)c->spill.p, (uintptr_t)c->scene_version,
+                                            (uintptr_t)c->lane.s, (uintptr_t)c->lane.spill.p};
         RCHK(c, run_graph(c->bgraph, key, c->stream, chain), "render: bounce graph");
     } else {
         RCHK(c, chain(), "render: bounce");

@@ -8,6 +8,7 @@
 #include "../../include/lfh.h"
 #include "capi_common.h"
 #include "dev_common.h"
+#include "pt_kernels.h"
 #include "pt_types.h"
 
 namespace lfh {
@@ -213,6 +214,42 @@ struct BounceGraph {
 // Bounce graphs are on unless LFH_BOUNCE_GRAPHS=0 (read once).
 bool bounce_graphs_enabled();
 
+// The stream, events and spill stack of a bounce chain's shadow lane
+// (ShadowLane).  Created with the first render; the stream has the default
+// priority.
+struct LaneRes {
+    hipStream_t s = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    DevBuf spill;
+    LaneRes() = default;
+    LaneRes(const LaneRes&) = delete;
+    LaneRes& operator=(const LaneRes&) = delete;
+    ~LaneRes() { release(); }
+    void release() {
+        for (hipEvent_t& e : ev) {
+            if (e) (void)hipEventDestroy(e);
+            e = nullptr;
+        }
+        if (s) (void)hipStreamDestroy(s);
+        s = nullptr;
+        spill.release();
+    }
+    // spill_bytes: the size of the main stream's spill stack
+    hipError_t init(size_t spill_bytes) {
+        hipError_t e = hipSuccess;
+        if (!s) e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+        for (hipEvent_t& v : ev)
+            if (e == hipSuccess && !v) e = hipEventCreateWithFlags(&v, hipEventDisableTiming);
+        if (e == hipSuccess && spill.bytes < spill_bytes) e = spill.alloc(spill_bytes);
+        if (e != hipSuccess) release();
+        return e;
+    }
+    bool ready() const { return s != nullptr; }
+    ShadowLane lane() const { return ShadowLane{s, spill.as<uint32_t>(), ev[0], ev[1]}; }
+};
+// The shadow lane is used unless LFH_SHADOW_LANE=0 (read once).
+bool shadow_lane_enabled();
+
 // One owner's guided rendering (a context, or a resident scene of the render
 // server): the guide, its training cadence, the passes rendered with it and
 // the buffers its wavefronts use.
@@ -265,5 +302,6 @@ struct lfh_ctx {
     // guiding
     lfh::GuideRun grun;
     lfh::BounceGraph bgraph;       // the unguided bounce chain of render_tile
+    lfh::LaneRes lane;             // render_tile's shadow lane
     uint64_t scene_version = 0;    // bumped by instance updates (the chain's scene view changes)
 };

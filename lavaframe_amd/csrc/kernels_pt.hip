@@ -100,6 +100,17 @@ __global__ void k_pt_advance(uint32_t* consumed, uint32_t* shadow_count, unsigne
     if (threadIdx.x < 2 * kFetchSlices) work[threadIdx.x * kFetchStride] = 0ull;
 }
 
+// One queue's half of k_pt_advance, for a chain whose shadow traversal runs
+// on its own stream: the queue's count goes to its running total, then the
+// count and the queue's fetch counters are cleared.
+__global__ void k_pt_advance_queue(uint32_t* count, unsigned long long* fetch, unsigned long long* total) {
+    if (threadIdx.x == 0) {
+        *total += *count;
+        *count = 0u;
+    }
+    if (threadIdx.x < kFetchSlices) fetch[threadIdx.x * kFetchStride] = 0ull;
+}
+
 // generate: one thread per tile pixel (MULTI: per pixel of every job)
 template <bool MULTI>
 __global__ __launch_bounds__(256) void k_generate(CamParams cam1, const JobDev* __restrict__ jobs, int njobs,
@@ -843,10 +854,37 @@ hipError_t pt_generate_rays(const float* rays6, const uint4* seeds, int n, const
     return hipGetLastError();
 }
 
+// Shading of `depth` waits for the lane's shadow traversal of depth - 1.
+static hipError_t wait_shadowed(const ShadowLane* lane, int depth, hipStream_t s) {
+    return lane && depth > 0 ? hipStreamWaitEvent(s, lane->shadowed, 0) : hipSuccess;
+}
+
+// After shading: the extension queue's counters are reset on the main stream,
+// and the shadow traversal with its reset runs on the lane.
+template <bool MULTI>
+static hipError_t shadow_on_lane(const ShadeScene& sc, const JobDev* jobs, uint32_t* count, float4* hitq,
+                                 float4* shadowq, uint32_t* shadow_count, const PathBufs& pb,
+                                 unsigned long long* work, int* err, float cull, int grid_blocks,
+                                 const ShadowLane& lane, hipStream_t s) {
+    hipLaunchKernelGGL(k_pt_advance_queue, dim3(1), dim3(64), 0, s, count, work, work + kWorkTotals);
+    hipError_t e = hipEventRecord(lane.shaded, s);
+    if (e == hipSuccess) e = hipStreamWaitEvent(lane.s, lane.shaded, 0);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_trace_queue<true, MULTI>), dim3(grid_blocks), dim3(kTraceBlock), 0, lane.s, sc, jobs,
+                       shadowq, shadow_count, hitq, pb, lane.spill, work + kFetchWords, cull, err);
+    hipLaunchKernelGGL(k_pt_advance_queue, dim3(1), dim3(64), 0, lane.s, shadow_count, work + kFetchWords,
+                       work + kWorkTotals + 1);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipEventRecord(lane.shadowed, lane.s);
+    return e;
+}
+
+hipError_t pt_bounce_join(const ShadowLane& lane, hipStream_t s) { return hipStreamWaitEvent(s, lane.shadowed, 0); }
+
 hipError_t pt_bounce(const ShadeScene& sc, int depth, int maxq, float4* rayq, uint32_t* count, float4* nextq,
                      uint32_t* next_count, float4* hitq, float4* shadowq, uint32_t* shadow_count, const PathBufs& pb,
                      uint32_t* spill, unsigned long long* work, int* err, float cull, int grid_blocks,
-                     const GuideHook& gh, const lfhd::GuideDev* gd, hipStream_t s) {
+                     const GuideHook& gh, const lfhd::GuideDev* gd, hipStream_t s, const ShadowLane* lane) {
     const JobDev* nj = nullptr;
     hipLaunchKernelGGL((k_trace_queue<false, false>), dim3(grid_blocks), dim3(kTraceBlock), 0, s, sc, nj, rayq, count,
                        hitq, pb, spill, work, cull, err);
@@ -855,8 +893,13 @@ hipError_t pt_bounce(const ShadeScene& sc, int depth, int maxq, float4* rayq, ui
                                          const_cast<float4*>(gh.lobes), s);
         if (e != hipSuccess) return e;
     }
+    hipError_t e = wait_shadowed(lane, depth, s);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_shade<false>, dim3(nblk(maxq)), dim3(256), 0, s, sc, nj, depth, rayq, hitq, count, nextq,
                        next_count, shadowq, shadow_count, pb, gh);
+    if (lane)
+        return shadow_on_lane<false>(sc, nj, count, hitq, shadowq, shadow_count, pb, work, err, cull, grid_blocks,
+                                     *lane, s);
     hipLaunchKernelGGL((k_trace_queue<true, false>), dim3(grid_blocks), dim3(kTraceBlock), 0, s, sc, nj, shadowq,
                        shadow_count, hitq, pb, spill, work + kFetchWords, cull, err);
     hipLaunchKernelGGL(k_pt_advance, dim3(1), dim3(64), 0, s, count, shadow_count, work);
@@ -867,7 +910,7 @@ hipError_t pt_bounce_multi(const JobDev* jobs, int depth, int maxq, float4* rayq
                            uint32_t* next_count, float4* hitq, float4* shadowq, uint32_t* shadow_count,
                            const PathBufs& pb, uint32_t* spill, unsigned long long* work, int* err, float cull,
                            int grid_blocks, const GuideHook& gh, const lfhd::GuideDev* gd, const ShadeScene* guide_sc,
-                           hipStream_t s) {
+                           hipStream_t s, const ShadowLane* lane) {
     ShadeScene dummy{};
     hipLaunchKernelGGL((k_trace_queue<false, true>), dim3(grid_blocks), dim3(kTraceBlock), 0, s, dummy, jobs, rayq,
                        count, hitq, pb, spill, work, cull, err);
@@ -877,8 +920,13 @@ hipError_t pt_bounce_multi(const JobDev* jobs, int depth, int maxq, float4* rayq
                                          const_cast<float4*>(gh.lobes), s);
         if (e != hipSuccess) return e;
     }
+    hipError_t e = wait_shadowed(lane, depth, s);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_shade<true>, dim3(nblk(maxq)), dim3(256), 0, s, dummy, jobs, depth, rayq, hitq, count, nextq,
                        next_count, shadowq, shadow_count, pb, gh);
+    if (lane)
+        return shadow_on_lane<true>(dummy, jobs, count, hitq, shadowq, shadow_count, pb, work, err, cull,
+                                    grid_blocks, *lane, s);
     hipLaunchKernelGGL((k_trace_queue<true, true>), dim3(grid_blocks), dim3(kTraceBlock), 0, s, dummy, jobs, shadowq,
                        shadow_count, hitq, pb, spill, work + kFetchWords, cull, err);
     hipLaunchKernelGGL(k_pt_advance, dim3(1), dim3(64), 0, s, count, shadow_count, work);

@@ -42,6 +42,18 @@ struct GuideHook {
 using lfhd::GuideHook;
 
 
+// Second stream of a wavefront's bounce chain.  The shadow-ray traversal of
+// depth d and its counter reset run on it, concurrently with the extension
+// traversal (and guide inference) of depth d + 1 on the main stream; shading
+// of depth d + 1 waits for them (it reads the occlusion results and appends to
+// the shadow queue again).  The two traversals use separate spill stacks.
+struct ShadowLane {
+    hipStream_t s = nullptr;
+    uint32_t* spill = nullptr;     // as large as the main stream's
+    hipEvent_t shaded = nullptr;   // recorded on the main stream after shading
+    hipEvent_t shadowed = nullptr; // recorded on s after the shadow traversal
+};
+
 hipError_t pt_reset(uint32_t* counters, unsigned long long* work, hipStream_t s);
 hipError_t pt_generate(const lfhd::CamParams& cam, const PathBufs& pb, float4* rayq, uint32_t* count, hipStream_t s);
 hipError_t pt_generate_rays(const float* rays6, const uint4* seeds, int n, const PathBufs& pb, float4* rayq,
@@ -49,7 +61,11 @@ hipError_t pt_generate_rays(const float* rays6, const uint4* seeds, int n, const
 hipError_t pt_bounce(const lfhd::ShadeScene& sc, int depth, int maxq, float4* rayq, uint32_t* count, float4* nextq,
                      uint32_t* next_count, float4* hitq, float4* shadowq, uint32_t* shadow_count, const PathBufs& pb,
                      uint32_t* spill, unsigned long long* work, int* err, float cull, int grid_blocks,
-                     const GuideHook& gh, const lfhd::GuideDev* gd, hipStream_t s);
+                     const GuideHook& gh, const lfhd::GuideDev* gd, hipStream_t s,
+                     const ShadowLane* lane = nullptr);
+// After the last depth of a chain that used `lane`: the main stream waits for
+// the lane's last shadow traversal.
+hipError_t pt_bounce_join(const ShadowLane& lane, hipStream_t s);
 // Multi-job pass (serving): every job renders its full frame once; rays of all
 // jobs share the queues.  jobs: device table of njobs entries.
 hipError_t pt_generate_multi(const lfhd::JobDev* jobs, int njobs, int total, const PathBufs& pb, float4* rayq,
@@ -59,7 +75,7 @@ hipError_t pt_bounce_multi(const lfhd::JobDev* jobs, int depth, int maxq, float4
                            float4* nextq, uint32_t* next_count, float4* hitq, float4* shadowq, uint32_t* shadow_count,
                            const PathBufs& pb, uint32_t* spill, unsigned long long* work, int* err, float cull,
                            int grid_blocks, const GuideHook& gh, const lfhd::GuideDev* gd,
-                           const lfhd::ShadeScene* guide_sc, hipStream_t s);
+                           const lfhd::ShadeScene* guide_sc, hipStream_t s, const ShadowLane* lane = nullptr);
 hipError_t pt_accumulate_multi(const lfhd::JobDev* jobs, int total, const PathBufs& pb, hipStream_t s);
 hipError_t pt_accumulate(const PathBufs& pb, const lfhd::CamParams& cam, float4* accum, float4* comp, int kahan,
                          hipStream_t s);

@@ -246,6 +246,7 @@ struct lfh_server {
     lfh::BounceGraph bgraph;               // the plain group's bounce chain
     int64_t gbuf_charged = 0;
     lfh::DevBuf table, spill, errflag;
+    lfh::LaneRes lane;                     // the bounce chain's shadow lane
     int grid = 0;
     int64_t used = 0;  // resident scenes + accum of admitted jobs + path shares of running jobs
     std::string err;
@@ -356,7 +357,9 @@ int lfh_server_create(int64_t budget_bytes, int32_t max_jobs, lfh_server** out) 
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->upload_stream, hipStreamNonBlocking);
     if (e == hipSuccess) {
         s->grid = pt_trace_grid_blocks();
-        e = s->spill.alloc((size_t)s->grid * lfh_trace_threads_per_block() * lfh_trace_spill_stride() * 4);
+        const size_t spill_bytes = (size_t)s->grid * lfh_trace_threads_per_block() * lfh_trace_spill_stride() * 4;
+        e = s->spill.alloc(spill_bytes);
+        if (e == hipSuccess && lfh::shadow_lane_enabled()) e = s->lane.init(spill_bytes);
     }
     if (e == hipSuccess) e = s->errflag.alloc(64);
     if (e == hipSuccess) e = hipMemsetAsync(s->errflag.p, 0, 64, s->stream);
@@ -396,6 +399,8 @@ void lfh_server_destroy(lfh_server* s) {
     s->table.release();
     s->spill.release();
     s->errflag.release();
+    if (s->lane.ready()) hipStreamSynchronize(s->lane.s);
+    s->lane.release();
     if (s->upload_stream) hipStreamDestroy(s->upload_stream);
     if (s->stream) hipStreamDestroy(s->stream);
     delete s;
@@ -557,6 +562,8 @@ static int run_group(lfh_server* s, const Group& g, const lfhd::JobDev* table_de
         if (lfh::guide_begin(ga, batch, &gh, &gd) < 0) return sfail(s, LFH_E_HIP, "lfh_server_step: guide buffers");
         gsc = lfh::shade_view(e.ps, e.ds, g.jobs[0]->opt);
     }
+    const ShadowLane lane = s->lane.lane();
+    const ShadowLane* lp = s->lane.ready() ? &lane : nullptr;
     auto chain = [&]() -> hipError_t {
         for (int depth = 0; depth < max_depth; ++depth) {
             const int cur = depth & 1;
@@ -564,16 +571,16 @@ static int run_group(lfh_server* s, const Group& g, const lfhd::JobDev* table_de
             hipError_t e = pt_bounce_multi(table_dev, depth, pb.cap, q[cur], cnt + cur, q[cur ^ 1], cnt + (cur ^ 1),
                                            pb.hq.as<float4>(), pb.sq.as<float4>(), cnt + 2, v, s->spill.as<uint32_t>(),
                                            work, s->errflag.as<int>(), 1.0f + 1.0f / 1024.0f, s->grid, gh, &gd, &gsc,
-                                           s->stream);
+                                           s->stream, lp);
             if (e != hipSuccess) return e;
         }
-        return hipSuccess;
+        return lp && max_depth > 0 ? pt_bounce_join(*lp, s->stream) : hipSuccess;
     };
     if (!g.guided_scene && lfh::bounce_graphs_enabled()) {
         // the plain group's chain reads its job table on the device: one
         // graph per table position, path capacity and depth
         const std::vector<uintptr_t> key = {(uintptr_t)table_dev, (uintptr_t)pb.rad.p, (uintptr_t)pb.cap,
-                                            (uintptr_t)max_depth, (uintptr_t)s->spill.p};
+                                            (uintptr_t)max_depth, (uintptr_t)s->spill.p, (uintptr_t)s->lane.s};
         SCHK(s, lfh::run_graph(s->bgraph, key, s->stream, chain), "lfh_server_step: bounce graph");
     } else {
         SCHK(s, chain(), "lfh_server_step: bounce");
