@@ -752,7 +752,7 @@ def _served_curve(sc, guided, px, mean, var_ref, max_spp, target, frame_offset=0
     return pts
 
 
-C3_REPEATS = 4   # independent random sequences (frame offsets) per estimator
+C3_REPEATS = 8   # independent random sequences (frame offsets) per estimator
 
 
 def guided_served_leg(stream, cpu=True):
