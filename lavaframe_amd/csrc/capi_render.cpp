@@ -142,13 +142,14 @@ int check_render_flag(lfh_ctx* c) {
 }
 
 // Paths per wavefront targeted by multi-sample passes over one-tile frames
-// (292 B of path state each, 4.9 GB at 2^24).  The last bounces of a
+// (292 B of path state each, 9.8 GB at 2^25).  The last bounces of a
 // wavefront hold few paths and cost a traversal's memory latency per launch,
 // so several passes share them.  2^23 -> 2^24: instanced 1080p scene 326 ->
 // 366 Msamples/s; guided / plain time on the guiding room 1.028 / 1.654 s ->
-// 0.967 / 1.470 s.
+// 0.967 / 1.470 s.  2^24 -> 2^25 (the 16 passes of the 1080p scene in one
+// wavefront instead of two): 416.4 / 417.1 -> 439.5 / 440.7 Msamples/s.
 #ifndef LFH_BATCH_PATHS_LOG2
-#define LFH_BATCH_PATHS_LOG2 24
+#define LFH_BATCH_PATHS_LOG2 25
 #endif
 constexpr int64_t kBatchPaths = (int64_t)1 << LFH_BATCH_PATHS_LOG2;
 constexpr int kMaxBatch = 64;
