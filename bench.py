@@ -718,20 +718,29 @@ def _guide_mlp_rate(guide, sc, stream, n=1 << 20, reps=10):
 C3_TARGETS = (1e-2, 5e-3, 2.5e-3)
 
 
-def _served_curve(sc, guided, px, mean, var_ref, max_spp, target, frame_offset=0):
+def _served_curve(sc, guided, px, mean, var_ref, max_spp, target, frame_offset=0, stand_in=None):
     """Serves the guiding room as one job (guided: the scene's guide, trained
     online by the job's wavefronts; frame_offset: an independent random
     sequence) and records after every server step the GPU wall time so far,
     the samples per pixel and the RMSE of the 1024 reference pixels against
     the FP64 reference (its own noise removed).  The clock pauses while the
-    image is read back.  The scene is made resident first, outside the timed
-    run, for both runs alike."""
+    image is read back.  Before the clock starts, for both runs alike, the
+    scene is made resident and the server's step buffers are sized as a
+    serving loop holds them between jobs: one full step of a job of the same
+    kind on `stand_in` (a scene of the same size and depth; a guided stand-in
+    trains its own guide, so the timed job's guide still starts from scratch)."""
     import torch
     from lavaframe_amd import lfh
     srv = lfh.Server(max_jobs=4)
     w = srv.submit_scene(sc, 1, estimator=1)
     srv.run_until_idle()
     srv.release(w)
+    if stand_in is not None:   # after the 1-spp job, which leaves the step buffers small
+        o = stand_in.options()
+        step_passes = max(1, (1 << 24) // (o.width * o.height))
+        w = srv.submit_scene(stand_in, step_passes, estimator=1, guiding=1 if guided else 0)
+        srv.run_until_idle()
+        srv.release(w)
     o = sc.options()
     jid = srv.submit_scene(sc, max_spp, estimator=1, guiding=1 if guided else 0, frame_offset=frame_offset)
     pts, spent = [], 0.0
@@ -752,7 +761,8 @@ def _served_curve(sc, guided, px, mean, var_ref, max_spp, target, frame_offset=0
     return pts
 
 
-C3_REPEATS = 8   # independent random sequences (frame offsets) per estimator
+C3_REPEATS = 8    # independent random sequences (frame offsets) per estimator, run to every target
+C3_REPEATS_1E2 = 32   # sequences, the first C3_REPEATS included, run to RMSE 1e-2 (the headline ratio)
 
 
 def guided_served_leg(stream, cpu=True):
@@ -762,10 +772,11 @@ def guided_served_leg(stream, cpu=True):
     online from scratch, 50/50 one-sample MIS), both with the consistent
     estimator.  Error = RMSE (radiance units) of the 1024 pixels of the FP64
     oracle's 65,536-spp reference in tests/golden, the reference's own noise
-    removed.  One run's error curve is noisy (the door's rare bright paths),
-    so each estimator runs C3_REPEATS times on independent random sequences;
-    reported: the median wall time to RMSE 1e-2 (and to 5e-3, 2.5e-3) of
-    each, their ratio guided / plain, and every run's times."""
+    removed.  One run's error curve is noisy (the door's rare bright paths:
+    single-run times to 1e-2 spread by +-20 %), so each estimator runs on
+    C3_REPEATS_1E2 independent random sequences to RMSE 1e-2, the first
+    C3_REPEATS of them on to 2.5e-3; reported: the median wall time to each
+    target, the ratio guided / plain, and every run's times."""
     from lavaframe_amd import scenes
     f = ROOT / "tests" / "golden" / "aperture_c3_fp64_64k.npz"
     if not f.exists():
@@ -776,19 +787,22 @@ def guided_served_leg(stream, cpu=True):
     std = z["std"].astype(np.float64)
     var_ref = float(np.mean(std ** 2)) / int(z["spp"])
     sc = scenes.aperture_room()
+    stand_in = scenes.aperture_room(seed=1)   # same size and depth, other furniture: its own guide
     target = min(C3_TARGETS)
     runs = {"plain": [], "guided": []}
-    for r in range(C3_REPEATS):
+    for r in range(max(C3_REPEATS, C3_REPEATS_1E2)):
         for kind in ("plain", "guided"):
-            runs[kind].append(_served_curve(sc, kind == "guided", px, mean, var_ref, 8192, target,
-                                            frame_offset=r << 20))
-    out = {"targets": {}, "repeats": C3_REPEATS,
+            runs[kind].append(_served_curve(sc, kind == "guided", px, mean, var_ref, 8192,
+                                            target if r < C3_REPEATS else max(C3_TARGETS), frame_offset=r << 20,
+                                            stand_in=stand_in))
+    out = {"targets": {}, "repeats": C3_REPEATS, "repeats_1e-2": C3_REPEATS_1E2,
            "metric": "median wall time to RMSE vs FP64 reference (1024 pixels), guided / plain, served",
            "reference": {"spp": int(z["spp"]), "pixels": len(px), "mean_radiance": round(float(mean.mean()), 6),
                          "noise_rms": round(math.sqrt(var_ref), 7)}}
     for tg in C3_TARGETS:
-        tp = [_time_to(c, tg) for c in runs["plain"]]
-        tgd = [_time_to(c, tg) for c in runs["guided"]]
+        n = C3_REPEATS_1E2 if tg == max(C3_TARGETS) else C3_REPEATS
+        tp = [_time_to(c, tg) for c in runs["plain"][:n]]
+        tgd = [_time_to(c, tg) for c in runs["guided"][:n]]
         mp = float(np.median([t for t, _ in tp]))
         mg = float(np.median([t for t, _ in tgd]))
         out["targets"][f"{tg:g}"] = {"plain_s": round(mp, 4), "guided_s": round(mg, 4), "ratio": round(mg / mp, 4),
@@ -799,7 +813,7 @@ def guided_served_leg(stream, cpu=True):
     plain, guided = runs["plain"][0], runs["guided"][0]
     out["plain_curve"] = [[s_, round(t, 4), round(e, 6)] for s_, t, e in plain[:: max(1, len(plain) // 24)]]
     out["guided_curve"] = [[s_, round(t, 4), round(e, 6)] for s_, t, e in guided[:: max(1, len(guided) // 24)]]
-    out["ms_per_pass"] = {k: round(1e3 * float(np.median([c[-1][1] / c[-1][0] for c in v])), 3)
+    out["ms_per_pass"] = {k: round(1e3 * float(np.median([c[-1][1] / c[-1][0] for c in v[:C3_REPEATS]])), 3)
                           for k, v in runs.items()}
     from lavaframe_amd import lfh
     out["guide_mlp"] = _guide_mlp_rate(lfh.Guide.for_scene(sc, seed=0), sc, stream)

@@ -50,6 +50,9 @@ namespace {
 // Device bytes of one pass's path state + queues per path (PassBufs).
 constexpr int64_t kPathBytes = 16 * (1 + 1 + 1 + 1 + 1 + 3 + 2 + 2 + 2 + 4) + 4;
 constexpr int64_t kAccumBytes = 32;  // sum + compensation, float4 each
+#ifndef LFH_SERVE_GUIDE_LR
+#define LFH_SERVE_GUIDE_LR 1e-3f   // Adam learning rate of a served scene's guide
+#endif
 #ifndef LFH_STEP_PATHS_LOG2
 #define LFH_STEP_PATHS_LOG2 24   // 60-job stream: 2^23 0.027 / 0.133 s p50 / p99, 2^24 0.026 / 0.116 s
 #endif
@@ -650,7 +653,7 @@ int lfh_server_step(lfh_server* s, int32_t* active) {
         if (ae == hipSuccess && j->guided && !e.guide) {
             // the scene's guide: created by its first guided job
             lfh_guide* gp = nullptr;
-            if (lfh_guide_create(e.lo, e.hi, 0, 1e-3f, &gp) != LFH_OK) ae = hipErrorOutOfMemory;
+            if (lfh_guide_create(e.lo, e.hi, 0, LFH_SERVE_GUIDE_LR, &gp) != LFH_OK) ae = hipErrorOutOfMemory;
             e.guide.reset(gp);
             e.grun.guide = gp;
             e.grun.train_every = 1;
