@@ -718,47 +718,58 @@ def _guide_mlp_rate(guide, sc, stream, n=1 << 20, reps=10):
 C3_TARGETS = (1e-2, 5e-3, 2.5e-3)
 
 
-def _served_curve(sc, guided, px, mean, var_ref, max_spp, target, frame_offset=0, stand_in=None):
-    """Serves the guiding room as one job (guided: the scene's guide, trained
-    online by the job's wavefronts; frame_offset: an independent random
-    sequence) and records after every server step the GPU wall time so far,
-    the samples per pixel and the RMSE of the 1024 reference pixels against
-    the FP64 reference (its own noise removed).  The clock pauses while the
-    image is read back.  Before the clock starts, for both runs alike, the
-    scene is made resident and the server's step buffers are sized as a
-    serving loop holds them between jobs: one full step of a job of the same
-    kind on `stand_in` (a scene of the same size and depth; a guided stand-in
-    trains its own guide, so the timed job's guide still starts from scratch)."""
-    import torch
-    from lavaframe_amd import lfh
-    srv = lfh.Server(max_jobs=4)
-    w = srv.submit_scene(sc, 1, estimator=1)
-    srv.run_until_idle()
-    srv.release(w)
-    if stand_in is not None:   # after the 1-spp job, which leaves the step buffers small
-        o = stand_in.options()
-        step_passes = max(1, (1 << 24) // (o.width * o.height))
-        w = srv.submit_scene(stand_in, step_passes, estimator=1, guiding=1 if guided else 0)
-        srv.run_until_idle()
-        srv.release(w)
-    o = sc.options()
-    jid = srv.submit_scene(sc, max_spp, estimator=1, guiding=1 if guided else 0, frame_offset=frame_offset)
-    pts, spent = [], 0.0
-    while True:
-        torch.cuda.synchronize()
-        t = time.perf_counter()
-        active = srv.step()
-        spent += time.perf_counter() - t
-        st = srv.status(jid)
-        if st.spp_done == 0:
-            continue
-        img = srv.result(jid, o.width, o.height).reshape(-1, 3)[px].astype(np.float64)
-        e = float(np.sqrt(max(float(np.mean((img - mean) ** 2)) - var_ref, 0.0)))
-        pts.append((int(st.spp_done), spent, e))
-        if e < target or st.state == 2 or active == 0:
-            break
-    srv.close()
-    return pts
+class _ServedRoom:
+    """One render server for one estimator, warmed once before any timed run,
+    as a serving loop holds it: the scene resident, and the step buffers sized
+    by one full step of a job of the same kind on `stand_in` (a scene of the
+    same size and depth).  Every timed run then serves the scene as one job
+    (guided: the scene's guide restarted from scratch with reset_guides and
+    trained online by the job's wavefronts; frame_offset: an independent
+    random sequence), records after every server step the GPU wall time so
+    far, the samples per pixel and the RMSE of the 1024 reference pixels
+    against the FP64 reference (its own noise removed), and is cancelled once
+    it reaches the target.  The clock pauses while the image is read back."""
+
+    def __init__(self, sc, guided, stand_in=None):
+        from lavaframe_amd import lfh
+        self.sc, self.guided = sc, guided
+        self.srv = lfh.Server(max_jobs=4)
+        w = self.srv.submit_scene(sc, 1, estimator=1)
+        self.srv.run_until_idle()
+        self.srv.release(w)
+        if stand_in is not None:   # after the 1-spp job, which leaves the step buffers small
+            o = stand_in.options()
+            step_passes = max(1, (1 << 24) // (o.width * o.height))
+            w = self.srv.submit_scene(stand_in, step_passes, estimator=1, guiding=1 if guided else 0)
+            self.srv.run_until_idle()
+            self.srv.release(w)
+
+    def curve(self, px, mean, var_ref, max_spp, target, frame_offset=0):
+        import torch
+        srv, o = self.srv, self.sc.options()
+        if self.guided:
+            srv.reset_guides()
+        jid = srv.submit_scene(self.sc, max_spp, estimator=1, guiding=1 if self.guided else 0,
+                               frame_offset=frame_offset)
+        pts, spent = [], 0.0
+        while True:
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            active = srv.step()
+            spent += time.perf_counter() - t
+            st = srv.status(jid)
+            if st.spp_done == 0:
+                continue
+            img = srv.result(jid, o.width, o.height).reshape(-1, 3)[px].astype(np.float64)
+            e = float(np.sqrt(max(float(np.mean((img - mean) ** 2)) - var_ref, 0.0)))
+            pts.append((int(st.spp_done), spent, e))
+            if e < target or st.state == 2 or active == 0:
+                break
+        srv.release(jid)
+        return pts
+
+    def close(self):
+        self.srv.close()
 
 
 C3_REPEATS = 8    # independent random sequences (frame offsets) per estimator, run to every target
@@ -790,11 +801,13 @@ def guided_served_leg(stream, cpu=True):
     stand_in = scenes.aperture_room(seed=1)   # same size and depth, other furniture: its own guide
     target = min(C3_TARGETS)
     runs = {"plain": [], "guided": []}
+    rooms = {kind: _ServedRoom(sc, kind == "guided", stand_in) for kind in runs}
     for r in range(max(C3_REPEATS, C3_REPEATS_1E2)):
         for kind in ("plain", "guided"):
-            runs[kind].append(_served_curve(sc, kind == "guided", px, mean, var_ref, 8192,
-                                            target if r < C3_REPEATS else max(C3_TARGETS), frame_offset=r << 20,
-                                            stand_in=stand_in))
+            runs[kind].append(rooms[kind].curve(px, mean, var_ref, 8192,
+                                                target if r < C3_REPEATS else max(C3_TARGETS), frame_offset=r << 20))
+    for room in rooms.values():
+        room.close()
     out = {"targets": {}, "repeats": C3_REPEATS, "repeats_1e-2": C3_REPEATS_1E2,
            "metric": "median wall time to RMSE vs FP64 reference (1024 pixels), guided / plain, served",
            "reference": {"spp": int(z["spp"]), "pixels": len(px), "mean_radiance": round(float(mean.mean()), 6),

@@ -388,7 +388,14 @@ int  lfh_server_set_policy(lfh_server* srv, int32_t policy, double nominal_rate,
 int  lfh_server_status(lfh_server* srv, int64_t job_id, lfh_job_status* st);
 /* Result of a finished job: linear HDR (w*h*3) like lfh_get_output_hdr. */
 int  lfh_server_result(lfh_server* srv, int64_t job_id, float* dst, int32_t w, int32_t h);
+/* Forgets a job and returns its memory.  A queued or running job is cancelled
+ * (its image is discarded); the server goes on with its other jobs. */
 int  lfh_server_release(lfh_server* srv, int64_t job_id);
+/* Drops the guides of the resident scenes: the next guided job of a scene
+ * starts a new guide from scratch (after the scene's lighting changed, or to
+ * time training from scratch on a warm server).  LFH_E_STATE while a guided
+ * job is queued or running.  No counterpart in the reference. */
+int  lfh_server_reset_guides(lfh_server* srv);
 /* Why a job was rejected (state -1): at submission (invalid scene, over the
  * budget) or at admission (scene upload or device BVH build failed, buffers
  * could not be allocated; the server goes on with its other jobs).  "" for

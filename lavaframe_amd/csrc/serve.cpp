@@ -888,14 +888,31 @@ int lfh_server_release(lfh_server* s, int64_t id) {
     auto it = s->jobs.find(id);
     if (it == s->jobs.end()) return sfail(s, LFH_E_INVALID, "lfh_server_release: unknown job");
     lfh::Job& j = *it->second;
-    if (j.state == 1) return sfail(s, LFH_E_STATE, "lfh_server_release: job still running");
+    // a running job is cancelled: the wavefronts that used its buffers have finished
     SCHK(s, hipStreamSynchronize(s->stream), "lfh_server_release: sync");
-    if (j.state == 2) {
+    if (j.state == 1 || j.state == 2) {
         std::lock_guard<std::mutex> lk(s->mu);
         s->used -= j.charged;
     }
     if (j.scene) j.scene->refs--;
     s->jobs.erase(it);
+    return LFH_OK;
+}
+
+int lfh_server_reset_guides(lfh_server* s) {
+    if (!s) return sfail(nullptr, LFH_E_INVALID, "lfh_server_reset_guides: NULL server");
+    for (auto& kv : s->jobs)
+        if (kv.second->guided && (kv.second->state == 0 || kv.second->state == 1))
+            return sfail(s, LFH_E_STATE, "lfh_server_reset_guides: a guided job is queued or running");
+    SCHK(s, hipStreamSynchronize(s->stream), "lfh_server_reset_guides: sync");
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (auto& kv : s->scenes) {
+        lfh::SceneEntry& e = *kv.second;
+        e.guide.reset();
+        e.grun.guide = nullptr;
+        e.grun.pass = 0;
+        e.grun.train_every = 0;
+    }
     return LFH_OK;
 }
 

@@ -182,6 +182,7 @@ _SIGS = {
     "lfh_get_ray_counts": (C.c_int, [_P, i64p, i64p]),
     "lfh_set_frame_offset": (C.c_int, [_P, C.c_int32]),
     "lfh_server_set_policy": (C.c_int, [_P, C.c_int32, C.c_double, C.c_double]),
+    "lfh_server_reset_guides": (C.c_int, [_P]),
     "lfh_guide_sample": (C.c_int, [_P, f32p, u32p, C.c_int64, f32p, f32p]),
     "lfh_guide_pdf": (C.c_int, [_P, f32p, f32p, C.c_int64, f32p]),
     "lfh_guide_samples_seen": (C.c_int, [_P, i64p]),
@@ -879,7 +880,12 @@ class Server:
         return out
 
     def release(self, job: int):
+        """Forgets a job; a queued or running job is cancelled."""
         self._chk(self.lib.lfh_server_release(self.h, job))
+
+    def reset_guides(self):
+        """The next guided job of every resident scene starts a new guide."""
+        self._chk(self.lib.lfh_server_reset_guides(self.h))
 
     def job_error(self, job: int) -> str:
         """Why a job was rejected ("" if it was not)."""

@@ -203,3 +203,64 @@ def test_step_policy_changes_scheduling_not_images():
         out.append(imgs)
     for a, b in zip(*out):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_releasing_a_running_job_cancels_it():
+    """A running job released between steps is cancelled; the other jobs go on
+    and their images are bitwise those of rendering them alone."""
+    print("[serve] cancel a running job", flush=True)
+    (a, na), (b, nb), _ = _jobs()
+    srv = lfh.Server(max_jobs=4)
+    ja = srv.submit_scene(a, 640)
+    jb = srv.submit_scene(b, nb)
+    for _ in range(50):
+        srv.step()
+        if srv.status(ja).state == 1:
+            break
+    assert srv.status(ja).state == 1
+    srv.release(ja)
+    with pytest.raises(lfh.LfhError):
+        srv.status(ja)
+    srv.run_until_idle()
+    st = srv.status(jb)
+    assert st.state == 2 and st.spp_done == nb
+    o = b.options()
+    img = srv.result(jb, o.width, o.height)
+    ref = _render_alone_ctx(b, nb)
+    assert np.array_equal(img.view(np.uint32), ref.reshape(img.shape).view(np.uint32))
+    srv.close()
+
+
+def test_reset_guides_restarts_guided_jobs_from_scratch():
+    """After reset_guides, a scene's next guided job trains a new guide: its
+    image is bitwise that of the same job on a fresh server, while without the
+    reset the job samples the guide its predecessor trained."""
+    print("[serve] reset guides", flush=True)
+    sc = scenes.cornell(32, 32)
+    o = sc.options()
+
+    def guided(srv, spp=24):
+        j = srv.submit_scene(sc, spp, estimator=1, guiding=1)
+        srv.run_until_idle()
+        assert srv.status(j).state == 2
+        img = srv.result(j, o.width, o.height)
+        srv.release(j)
+        return img
+
+    fresh = lfh.Server(max_jobs=2)
+    ref = guided(fresh)
+    fresh.close()
+    srv = lfh.Server(max_jobs=2)
+    first = guided(srv)
+    assert np.array_equal(first.view(np.uint32), ref.view(np.uint32))
+    warm = guided(srv)   # samples the guide the first job trained
+    assert not np.array_equal(warm.view(np.uint32), ref.view(np.uint32))
+    srv.reset_guides()
+    again = guided(srv)
+    assert np.array_equal(again.view(np.uint32), ref.view(np.uint32))
+    j = srv.submit_scene(sc, 8, estimator=1, guiding=1)
+    with pytest.raises(lfh.LfhError):
+        srv.reset_guides()   # a guided job is queued
+    srv.run_until_idle()
+    srv.release(j)
+    srv.close()
