@@ -45,7 +45,9 @@ constexpr int kRecordedPaths = 1 << LFH_GUIDE_RECORD_LOG2;     // recorded paths
 // 4/128/32 0.195 s, 4/64/32 0.174, 8/128/32 0.180, 8/64/32 0.167, 4/128/128
 // 0.161-0.177, 4/64/128 0.173, 8/64/128 0.175, 8/128/128 0.188, 4/128/64
 // 0.191, 8/64/64 0.173, 4/128/1024 0.192; serial warm-up 0.308.  The spread
-// of one median over different run sets is about +-0.015 s.
+// of one median over different run sets is about +-0.015 s.  Re-measured on one
+// warm server over 32 sequences (learning rate 1e-3): 64 steps 0.168 s, 128 steps
+// 0.162; with rate 2e-3: 32 steps 0.160, 64 steps 0.161, 96 steps 0.161, 128 steps 0.168.
 #define LFH_GUIDE_WARM_STEPS 64
 #endif
 #ifndef LFH_GUIDE_WARM_REPEATS
@@ -56,7 +58,9 @@ constexpr int kRecordedPaths = 1 << LFH_GUIDE_RECORD_LOG2;     // recorded paths
 // the guide; 0: serial warm-up (one pass per wavefront until kWarmSteps).
 // Served guiding room, median guided time to RMSE 1e-2 over 8 runs (64 Adam
 // steps, cap 32): serial 0.308 s, 4 passes 0.174 s, 8 passes 0.167 s; with cap
-// 128: 4 passes 0.173 s, 8 passes 0.175 s.
+// 128: 4 passes 0.173 s, 8 passes 0.175 s.  On one warm server over 32 sequences
+// (learning rate 1e-3): 2 passes 0.166 s, 4 passes 0.168, 8 passes 0.169; with
+// rate 2e-3: 2 passes 0.152 (0.157 on a second set), 3 passes 0.165, 8 passes 0.170.
 #define LFH_GUIDE_PLAIN_WARMUP 4
 #endif
 #ifndef LFH_GUIDE_LATE_REPEATS
@@ -73,7 +77,8 @@ constexpr float kWeightCap = LFH_GUIDE_WEIGHT_CAP;   // online training: weights
 // Served guiding room, median over 6 runs of the guided time to RMSE 1e-2 / 5e-3:
 // every pass 0.156 / 0.606 s; every 2nd pass after 128 steps 0.156 / 0.586 s
 // (2.106 ms per pass against 2.12); every 2nd or 4th pass after 64 steps 0.167 /
-// 0.657 s and 0.187 / 0.624 s.
+// 0.657 s and 0.187 / 0.624 s.  On one warm server over 32 sequences to 1e-2:
+// every pass 0.168 s, every 2nd after 128 steps 0.168 s.
 #define LFH_GUIDE_LATE_EVERY 2
 #endif
 #ifndef LFH_GUIDE_LATE_AFTER
@@ -81,6 +86,14 @@ constexpr float kWeightCap = LFH_GUIDE_WEIGHT_CAP;   // online training: weights
 #endif
 constexpr int kLateEvery = LFH_GUIDE_LATE_EVERY;   // once kLateAfter steps are taken, one training pass in kLateEvery
 constexpr int kLateAfter = LFH_GUIDE_LATE_AFTER;
+#ifndef LFH_GUIDE_LR_LATE
+#define LFH_GUIDE_LR_LATE 1.0f
+#endif
+#ifndef LFH_GUIDE_LR_DECAY_AFTER
+#define LFH_GUIDE_LR_DECAY_AFTER 256
+#endif
+constexpr float kLrLate = LFH_GUIDE_LR_LATE;            // learning-rate factor once kLrDecayAfter steps are taken
+constexpr int kLrDecayAfter = LFH_GUIDE_LR_DECAY_AFTER;
 }  // namespace
 
 struct lfh_guide {
@@ -112,7 +125,7 @@ struct lfh_guide {
         a.wf = wf.as<uint16_t>();
         a.wb = wb.as<uint16_t>();
         a.bias = bias.as<float>();
-        a.lr = lr;
+        a.lr = steps >= kLrDecayAfter ? lr * kLrLate : lr;
         a.b1 = 0.9f;
         a.b2 = 0.999f;
         a.eps = 1e-8f;

@@ -51,7 +51,10 @@ namespace {
 constexpr int64_t kPathBytes = 16 * (1 + 1 + 1 + 1 + 1 + 3 + 2 + 2 + 2 + 4) + 4;
 constexpr int64_t kAccumBytes = 32;  // sum + compensation, float4 each
 #ifndef LFH_SERVE_GUIDE_LR
-#define LFH_SERVE_GUIDE_LR 1e-3f   // Adam learning rate of a served scene's guide
+// Adam learning rate of a served scene's guide.  Guiding room on one warm server, median
+// guided time to RMSE 1e-2 over 32 random sequences (a second set of 32 in brackets):
+// 1e-3 0.168 (0.162) s, 1.5e-3 0.156 (0.155), 2e-3 0.161 (0.159), 2.5e-3 0.163, 3e-3 0.164.
+#define LFH_SERVE_GUIDE_LR 1.5e-3f
 #endif
 #ifndef LFH_STEP_PATHS_LOG2
 #define LFH_STEP_PATHS_LOG2 24   // 60-job stream: 2^23 0.027 / 0.133 s p50 / p99, 2^24 0.026 / 0.116 s
