@@ -407,7 +407,8 @@ double lfh_server_clock(const lfh_server* srv);
 typedef struct lfh_guide lfh_guide;
 /* MLP: 64-dim frequency encoding of (position, normal, wo) -> 3 x 64 ReLU ->
  * 8 vMF lobes (mu 3, log kappa 1, weight logit 1 each = 40 outputs).  Weights
- * BF16, accumulation FP32, Adam. */
+ * BF16, accumulation FP32, Adam at learning_rate, halved after 160 steps.  A
+ * render server's guides use 1.5e-3. */
 int  lfh_guide_create(const float* scene_min3, const float* scene_max3, uint64_t seed,
                       float learning_rate, lfh_guide** out);
 void lfh_guide_destroy(lfh_guide* g);

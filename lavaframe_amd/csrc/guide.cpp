@@ -86,11 +86,17 @@ constexpr float kWeightCap = LFH_GUIDE_WEIGHT_CAP;   // online training: weights
 #endif
 constexpr int kLateEvery = LFH_GUIDE_LATE_EVERY;   // once kLateAfter steps are taken, one training pass in kLateEvery
 constexpr int kLateAfter = LFH_GUIDE_LATE_AFTER;
+// The guide learns fastest early at a higher rate, and refines better later at a lower
+// one.  Served guiding room (rate 1.5e-3), median guided time to RMSE 1e-2 / 5e-3 /
+// 2.5e-3 over 8 sequences on one warm server: constant 0.143 / 0.571-0.573 / 2.41-2.42 s;
+// halved after 160 steps 0.143 / 0.494 / 2.32; halved after 128 0.144 / 0.552 / 2.40;
+// halved after 256 0.143 / 0.568 / 2.22; a third after 160 0.143 / 0.513 / 2.26; constant
+// 1e-3: 0.154 / 0.604 / 2.16.
 #ifndef LFH_GUIDE_LR_LATE
-#define LFH_GUIDE_LR_LATE 1.0f
+#define LFH_GUIDE_LR_LATE 0.5f
 #endif
 #ifndef LFH_GUIDE_LR_DECAY_AFTER
-#define LFH_GUIDE_LR_DECAY_AFTER 256
+#define LFH_GUIDE_LR_DECAY_AFTER 160
 #endif
 constexpr float kLrLate = LFH_GUIDE_LR_LATE;            // learning-rate factor once kLrDecayAfter steps are taken
 constexpr int kLrDecayAfter = LFH_GUIDE_LR_DECAY_AFTER;
