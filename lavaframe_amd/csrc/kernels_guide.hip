@@ -30,6 +30,12 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kPitch = 72;            // BF16 elements per activation row
 constexpr int kOutPitch = 41;         // FP32 elements per output row
+#ifndef LFH_INFER_TILES
+// 16-query tiles a wave runs through the MLP together (1 or 2).  Two (218 VGPRs, still 2
+// waves per SIMD) measured 576 against 588 TFLOP/s on 1M host queries and the same guided
+// C3 time: the four independent accumulator chains of one tile already fill the MFMA pipe.
+#define LFH_INFER_TILES 1
+#endif
 constexpr int kInferBlock = 256;      // 4 persistent waves
 constexpr int kInferWaves = kInferBlock / 64;
 constexpr int kTrainRows = 64;        // samples per training block
@@ -193,28 +199,8 @@ __global__ __launch_bounds__(kInferBlock) void k_guide_infer(GuideDev g, const f
         // the rows are read back by other lanes of this wave only: LDS
         // operations of one wave complete in order
         encode_row(X + lane * kPitch, g, pos, nrm, wo);
-#pragma unroll 1
-        for (int t = 0; t < 4; ++t) {
-            if (base + t * 16 >= n) break;   // wave-uniform
-            const uint16_t* xr = X + (t * 16 + lr) * kPitch + grp * 8;
-            bf16x8 b0 = ld_frag(xr), b1 = ld_frag(xr + 32);
-            f32x4 acc[4];
-#pragma unroll
-            for (int l = 0; l < kGuideLayers; ++l) {
-                const int nN = l < 3 ? 4 : 3;
-#pragma unroll
-                for (int nt = 0; nt < 4; ++nt) {
-                    if (nt >= nN) continue;
-                    const float4 bb = sBias[(l * 64 + nt * 16 + grp * 4) >> 2];
-                    acc[nt] = f32x4{bb.x, bb.y, bb.z, bb.w};
-                    acc[nt] = mfma(w[l][nt][0], b0, acc[nt]);
-                    acc[nt] = mfma(w[l][nt][1], b1, acc[nt]);
-                }
-                if (l < 3) {
-                    b0 = next_operand(acc[0], acc[1]);
-                    b1 = next_operand(acc[2], acc[3]);
-                }
-            }
+        // Outputs of one 16-query tile: raw40 and / or the processed lobes.
+        auto emit = [&](const f32x4 (&acc)[4], int t) {
             // outputs 10 grp + s, s = 4 nt + i, in acc[nt][i]
             float o[10];
 #pragma unroll
@@ -225,7 +211,7 @@ __global__ __launch_bounds__(kInferBlock) void k_guide_infer(GuideDev g, const f
 #pragma unroll
                 for (int s2 = 0; s2 < 10; ++s2) dst[s2] = o[s2];
             }
-            if (!lobes) continue;
+            if (!lobes) return;
             // the processed lobes (dev_guide.h) over the query's four lane
             // groups: lobes 2 grp, 2 grp + 1 here
             float m = fmaxf(o[4], o[9]);
@@ -254,7 +240,60 @@ __global__ __launch_bounds__(kInferBlock) void k_guide_infer(GuideDev g, const f
                 dst[(2 * grp + 1) * st] = lb[1];
                 reinterpret_cast<float2*>(dst + (8 + (grp >> 1)) * st)[grp & 1] = make_float2(wk[0], wk[1]);
             }
+        };
+        // The four layers of one tile from its two B fragments.
+        auto layer = [&](int l, bf16x8& b0, bf16x8& b1, f32x4 (&acc)[4]) {
+            const int nN = l < 3 ? 4 : 3;
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) {
+                if (nt >= nN) continue;
+                const float4 bb = sBias[(l * 64 + nt * 16 + grp * 4) >> 2];
+                acc[nt] = f32x4{bb.x, bb.y, bb.z, bb.w};
+                acc[nt] = mfma(w[l][nt][0], b0, acc[nt]);
+                acc[nt] = mfma(w[l][nt][1], b1, acc[nt]);
+            }
+        };
+#if LFH_INFER_TILES == 2
+        // two tiles at a time: each one's MFMA chain overlaps the other's packing
+#pragma unroll 1
+        for (int t = 0; t < 4; t += 2) {
+            if (base + t * 16 >= n) break;   // wave-uniform
+            const uint16_t* xa = X + (t * 16 + lr) * kPitch + grp * 8;
+            const uint16_t* xb = xa + 16 * kPitch;
+            bf16x8 a0 = ld_frag(xa), a1 = ld_frag(xa + 32), c0 = ld_frag(xb), c1 = ld_frag(xb + 32);
+            f32x4 accA[4], accB[4];
+#pragma unroll
+            for (int l = 0; l < kGuideLayers; ++l) {
+                layer(l, a0, a1, accA);
+                layer(l, c0, c1, accB);
+                if (l < 3) {
+                    a0 = next_operand(accA[0], accA[1]);
+                    a1 = next_operand(accA[2], accA[3]);
+                    c0 = next_operand(accB[0], accB[1]);
+                    c1 = next_operand(accB[2], accB[3]);
+                }
+            }
+            emit(accA, t);
+            emit(accB, t + 1);
         }
+#else
+#pragma unroll 1
+        for (int t = 0; t < 4; ++t) {
+            if (base + t * 16 >= n) break;   // wave-uniform
+            const uint16_t* xr = X + (t * 16 + lr) * kPitch + grp * 8;
+            bf16x8 b0 = ld_frag(xr), b1 = ld_frag(xr + 32);
+            f32x4 acc[4];
+#pragma unroll
+            for (int l = 0; l < kGuideLayers; ++l) {
+                layer(l, b0, b1, acc);
+                if (l < 3) {
+                    b0 = next_operand(acc[0], acc[1]);
+                    b1 = next_operand(acc[2], acc[3]);
+                }
+            }
+            emit(acc, t);
+        }
+#endif
         rec = next;
     }
 }

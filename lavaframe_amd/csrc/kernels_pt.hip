@@ -854,6 +854,16 @@ hipError_t pt_generate_rays(const float* rays6, const uint4* seeds, int n, const
     return hipGetLastError();
 }
 
+#ifndef LFH_LANE_GRID_NUM
+// the lane's shadow traversal launches this many quarters of the resident grid: 2 / 3 / 4 quarters
+// measured C0 1287.8 / 1255.5 / 1228.6 Msamples/s, C2 429.0 / 438.9 / 440.0, guided C3 pass
+// 2.217 / 2.113 / 2.072 ms
+#define LFH_LANE_GRID_NUM 4
+#endif
+#ifndef LFH_INFER_AFTER_SHADOW
+#define LFH_INFER_AFTER_SHADOW 0   // 1: guide inference waits for the shadow traversal too (median
+                                   // guided C3 time to 1e-2 over 32 sequences 0.158 s against 0.156)
+#endif
 // Shading of `depth` waits for the lane's shadow traversal of depth - 1.
 static hipError_t wait_shadowed(const ShadowLane* lane, int depth, hipStream_t s) {
     return lane && depth > 0 ? hipStreamWaitEvent(s, lane->shadowed, 0) : hipSuccess;
@@ -870,7 +880,8 @@ static hipError_t shadow_on_lane(const ShadeScene& sc, const JobDev* jobs, uint3
     hipError_t e = hipEventRecord(lane.shaded, s);
     if (e == hipSuccess) e = hipStreamWaitEvent(lane.s, lane.shaded, 0);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_trace_queue<true, MULTI>), dim3(grid_blocks), dim3(kTraceBlock), 0, lane.s, sc, jobs,
+    const int lane_blocks = grid_blocks * LFH_LANE_GRID_NUM / 4 > 0 ? grid_blocks * LFH_LANE_GRID_NUM / 4 : 1;
+    hipLaunchKernelGGL((k_trace_queue<true, MULTI>), dim3(lane_blocks), dim3(kTraceBlock), 0, lane.s, sc, jobs,
                        shadowq, shadow_count, hitq, pb, lane.spill, work + kFetchWords, cull, err);
     hipLaunchKernelGGL(k_pt_advance_queue, dim3(1), dim3(64), 0, lane.s, shadow_count, work + kFetchWords,
                        work + kWorkTotals + 1);
@@ -888,6 +899,10 @@ hipError_t pt_bounce(const ShadeScene& sc, int depth, int maxq, float4* rayq, ui
     const JobDev* nj = nullptr;
     hipLaunchKernelGGL((k_trace_queue<false, false>), dim3(grid_blocks), dim3(kTraceBlock), 0, s, sc, nj, rayq, count,
                        hitq, pb, spill, work, cull, err);
+    if (LFH_INFER_AFTER_SHADOW && gh.enabled) {
+        hipError_t e = wait_shadowed(lane, depth, s);
+        if (e != hipSuccess) return e;
+    }
     if (gh.enabled) {
         hipError_t e = guide_infer_queue(*gd, sc, rayq, hitq, count, maxq, gh, depth,
                                          const_cast<float4*>(gh.lobes), s);
@@ -914,6 +929,10 @@ hipError_t pt_bounce_multi(const JobDev* jobs, int depth, int maxq, float4* rayq
     ShadeScene dummy{};
     hipLaunchKernelGGL((k_trace_queue<false, true>), dim3(grid_blocks), dim3(kTraceBlock), 0, s, dummy, jobs, rayq,
                        count, hitq, pb, spill, work, cull, err);
+    if (LFH_INFER_AFTER_SHADOW && gh.enabled) {
+        hipError_t e = wait_shadowed(lane, depth, s);
+        if (e != hipSuccess) return e;
+    }
     if (gh.enabled) {
         // every job of a guided wavefront draws on one scene (the guide's)
         hipError_t e = guide_infer_queue(*gd, *guide_sc, rayq, hitq, count, maxq, gh, depth,
