@@ -28,6 +28,8 @@ constexpr int kGuideParams = 3 * (64 * 64 + 64) + 64 * 40 + 40;  // 15080
 constexpr float kLogKMin = -3.0f;
 constexpr float kLogKMax = 7.0f;
 #ifndef LFH_GUIDE_FRAC
+// Served guiding room, median guided time to RMSE 1e-2 over two sets of 32 sequences:
+// 0.35: 0.165 / 0.161 s, 0.5: 0.156 / 0.155, 0.65: 0.164 / 0.166, 0.75: 0.158 / 0.160.
 #define LFH_GUIDE_FRAC 0.5f
 #endif
 constexpr float kGuideFrac = LFH_GUIDE_FRAC;   // probability of drawing from the guide
