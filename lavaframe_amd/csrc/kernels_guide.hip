@@ -30,6 +30,12 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kPitch = 72;            // BF16 elements per activation row
 constexpr int kOutPitch = 41;         // FP32 elements per output row
+#ifndef LFH_INFER_LDS_W
+// 1: layers 0 and 3 of the inference weights in LDS (52 KB per block), budgeted for 3 waves per
+// SIMD (162 VGPRs).  Measured 575 against 600 TFLOP/s on 1M host queries and a 1 % slower guided
+// C3 pass: the per-MFMA LDS reads cost more than the third wave hides.
+#define LFH_INFER_LDS_W 0
+#endif
 #ifndef LFH_INFER_TILES
 // 16-query tiles a wave runs through the MLP together (1 or 2).  Two (218 VGPRs, still 2
 // waves per SIMD) measured 576 against 588 TFLOP/s on 1M host queries and the same guided
@@ -146,11 +152,20 @@ LFH_DEV bf16x8 next_operand(f32x4 lo, f32x4 hi) {
 // raw40 (40 outputs per query) and/or the processed lobes (dev_guide.h;
 // QUEUE: structure of arrays with stride gh.lobe_stride).
 template <bool QUEUE>
-__global__ __launch_bounds__(kInferBlock) void k_guide_infer(GuideDev g, const float* __restrict__ in9, int n_host,
+__global__ __launch_bounds__(kInferBlock, LFH_INFER_LDS_W ? 3 : 1) void k_guide_infer(GuideDev g, const float* __restrict__ in9, int n_host,
                                                              InferQueue q, float* __restrict__ raw40,
                                                              float4* __restrict__ lobes) {
     __shared__ __attribute__((aligned(16))) uint16_t sX[kInferWaves][64 * kPitch];
     __shared__ float4 sBias[kGuideBImage / 4];
+#if LFH_INFER_LDS_W
+    // layers 0 and 3 (14 fragments) from LDS, layers 1 and 2 in registers
+    __shared__ bf16x8 sW[14][64];
+    for (int i = threadIdx.x; i < 14 * 64; i += kInferBlock) {
+        const int f = i >> 6, ln = i & 63;
+        const int l = f < 8 ? 0 : 3, nt = (f & 7) >> 1, ks = f & 1;
+        sW[f][ln] = ld_frag(g.wf + kGuideWImage + ((l * 4 + nt) * 2 + ks) * 512 + ln * 8);
+    }
+#endif
     const uint32_t n = QUEUE ? *q.count : (uint32_t)n_host;
     if ((uint32_t)blockIdx.x * kInferBlock >= n) return;  // block-uniform
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -166,7 +181,9 @@ __global__ __launch_bounds__(kInferBlock) void k_guide_infer(GuideDev g, const f
             for (int nt = 0; nt < 4; ++nt)
 #pragma unroll
                 for (int ks = 0; ks < 2; ++ks)
-                    w[l][nt][ks] = (l < 3 || nt < 3) ? ld_frag(wi + ((l * 4 + nt) * 2 + ks) * 512) : bf16x8{};
+                    w[l][nt][ks] = (l < 3 || nt < 3) && !(LFH_INFER_LDS_W && (l == 0 || l == 3))
+                                       ? ld_frag(wi + ((l * 4 + nt) * 2 + ks) * 512)
+                                       : bf16x8{};
     }
     __syncthreads();
     uint16_t* X = sX[wave];
@@ -249,6 +266,14 @@ __global__ __launch_bounds__(kInferBlock) void k_guide_infer(GuideDev g, const f
                 if (nt >= nN) continue;
                 const float4 bb = sBias[(l * 64 + nt * 16 + grp * 4) >> 2];
                 acc[nt] = f32x4{bb.x, bb.y, bb.z, bb.w};
+#if LFH_INFER_LDS_W
+                if (l == 0 || l == 3) {
+                    const int f = (l == 0 ? 0 : 8) + nt * 2;
+                    acc[nt] = mfma(sW[f][lane], b0, acc[nt]);
+                    acc[nt] = mfma(sW[f + 1][lane], b1, acc[nt]);
+                    continue;
+                }
+#endif
                 acc[nt] = mfma(w[l][nt][0], b0, acc[nt]);
                 acc[nt] = mfma(w[l][nt][1], b1, acc[nt]);
             }
