@@ -33,7 +33,7 @@ constexpr float kLogKMax = 7.0f;
 #define LFH_GUIDE_FRAC 0.5f
 #endif
 constexpr float kGuideFrac = LFH_GUIDE_FRAC;   // probability of drawing from the guide
-constexpr int kGuideLobeF4 = 10;     // processed lobes per query: 8 x (mu, kappa), 2 x (pi * vmf_norm)[4]
+constexpr int kGuideLobeF4 = 6;      // processed lobes per query: 4 x two packed (mu, kappa), 2 x (pi * vmf_norm)[4]
 
 LFH_HD int guide_layer_out(int l) { return l == 3 ? kGuideOut : kGuideHid; }
 LFH_HD int guide_layer_offset(int l) { return l * (64 * 64 + 64); }
@@ -121,12 +121,13 @@ LFH_DEV float vmf_norm(float kappa) {
 // 1 / vmf_norm, for the sampler's choice of lobe.
 LFH_DEV float vmf_inv_norm(float kappa) { return kTwoPi * (1.0f - __expf(-2.0f * kappa)) * __builtin_amdgcn_rcpf(kappa); }
 
-// Processed lobes, kGuideLobeF4 float4 per query: float4 k < 8 is lobe k's
-// (mu, kappa) from outputs 5k..5k+4 = (r, s, a), mu = normalize(r), kappa =
-// exp(clamp(s, kLogKMin, kLogKMax)); float4 8 and 9 hold w_k = pi_k *
-// vmf_norm(kappa_k), pi = softmax(a) (the factor every pdf evaluation needs;
-// the sampler divides it out again), or -1 for a query whose outputs are not
-// all finite (no guide).  guide_lobe makes one (mu, kappa).
+// Processed lobes, kGuideLobeF4 float4 per query: float4 r < 4 holds lobes 2r
+// and 2r+1 packed (guide_pack_lobe), each the (mu, kappa) of outputs 5k..5k+4
+// = (r, s, a), mu = normalize(r), kappa = exp(clamp(s, kLogKMin, kLogKMax));
+// float4 4 and 5 hold w_k = pi_k * vmf_norm(kappa_k) of the packed kappa, pi =
+// softmax(a) (the factor every pdf evaluation needs; the sampler divides it
+// out again), or -1 for a query whose outputs are not all finite (no guide).
+// guide_lobe makes one (mu, kappa).
 LFH_DEV float4 guide_lobe(const float* o5) {
     float rx = o5[0], ry = o5[1], rz = o5[2];
     const float r2 = rx * rx + ry * ry + rz * rz;
@@ -136,10 +137,33 @@ LFH_DEV float4 guide_lobe(const float* o5) {
     return make_float4(rx * ir, ry * ir, rz * ir, __expf(sk));
 }
 
-// The processed lobes of one query are lob[k * st], k = 0..9 (st = 1: ten
+// A lobe in 8 bytes: mu.x, mu.y, mu.z and kappa as binary16, rounded to
+// nearest even.  The sampler and every pdf evaluation read the lobe back
+// through guide_unpack_lobe, so the mixture they use is one and the same.
+LFH_DEV uint2 guide_pack_lobe(float4 l) {
+    const uint32_t x = __builtin_bit_cast(uint16_t, (_Float16)l.x), y = __builtin_bit_cast(uint16_t, (_Float16)l.y);
+    const uint32_t z = __builtin_bit_cast(uint16_t, (_Float16)l.z), k = __builtin_bit_cast(uint16_t, (_Float16)l.w);
+    return make_uint2(x | (y << 16), z | (k << 16));
+}
+LFH_DEV float half_lo(uint32_t v) { return (float)__builtin_bit_cast(_Float16, (uint16_t)(v & 0xFFFFu)); }
+LFH_DEV float half_hi(uint32_t v) { return (float)__builtin_bit_cast(_Float16, (uint16_t)(v >> 16)); }
+LFH_DEV float guide_packed_kappa(uint2 p) { return half_hi(p.y); }
+// (mu renormalised after the rounding, kappa)
+LFH_DEV float4 guide_unpack_lobe(uint2 p) {
+    const float x = half_lo(p.x), y = half_hi(p.x), z = half_lo(p.y);
+    const float ir = __builtin_amdgcn_rsqf(x * x + y * y + z * z);
+    return make_float4(x * ir, y * ir, z * ir, half_hi(p.y));
+}
+// Packed lobe j of one query (lob as in guide_valid).
+LFH_DEV uint2 guide_packed(const float4* __restrict__ lob, int j, size_t st) {
+    const uint4 u = __builtin_bit_cast(uint4, lob[(j >> 1) * st]);
+    return (j & 1) ? make_uint2(u.z, u.w) : make_uint2(u.x, u.y);
+}
+
+// The processed lobes of one query are lob[r * st], r = 0..5 (st = 1: six
 // consecutive float4; larger st: one query of a structure-of-arrays buffer).
 // True when they are usable.
-LFH_DEV bool guide_valid(const float4* __restrict__ lob, size_t st = 1) { return lob[8 * st].x >= 0.0f; }
+LFH_DEV bool guide_valid(const float4* __restrict__ lob, size_t st = 1) { return lob[4 * st].x >= 0.0f; }
 
 // Inference image.  Inference runs the MLP transposed, Y^T = W^T X^T, on
 // 16-query tiles with v_mfma_f32_16x16x32_bf16 (weights = A operand, 16 rows
@@ -191,12 +215,12 @@ LFH_DEV void guide_store_param(uint16_t* wf, uint16_t* wb, float* bias, int j, f
 // sum_k pi_k vmf_norm(kappa_k) exp(kappa_k (mu_k . d - 1)), the first product
 // stored per lobe in the processed lobes.
 LFH_DEV float guide_pdf(const float4* __restrict__ lob, V3 d, size_t st = 1) {
-    const float4 pa = lob[8 * st], pb = lob[9 * st];
+    const float4 pa = lob[4 * st], pb = lob[5 * st];
     const float w[8] = {pa.x, pa.y, pa.z, pa.w, pb.x, pb.y, pb.z, pb.w};
     float p = 0.0f;
 #pragma unroll
     for (int k = 0; k < kGuideLobes; ++k) {
-        const float4 l = lob[k * st];
+        const float4 l = guide_unpack_lobe(guide_packed(lob, k, st));
         const float c = l.x * d.x + l.y * d.y + l.z * d.z;
         p += w[k] * __expf(l.w * (c - 1.0f));
     }
@@ -218,11 +242,15 @@ LFH_DEV void guide_frame(V3 n, V3& t, V3& b) {
 // von Mises Fisher distribution on S2"): w = 1 + log(u1 + (1-u1) e^{-2 kappa}) / kappa,
 // phi = 2 pi u2.
 LFH_DEV V3 guide_sample(const float4* __restrict__ lob, float u0, float u1, float u2, size_t st = 1) {
-    const float4 pa = lob[8 * st], pb = lob[9 * st];
+    const float4 pa = lob[4 * st], pb = lob[5 * st];
     const float wn[8] = {pa.x, pa.y, pa.z, pa.w, pb.x, pb.y, pb.z, pb.w};
+    uint2 pk[8];
     float pis[8];
 #pragma unroll
-    for (int j = 0; j < kGuideLobes; ++j) pis[j] = wn[j] * vmf_inv_norm(lob[j * st].w);
+    for (int j = 0; j < kGuideLobes; ++j) {
+        pk[j] = guide_packed(lob, j, st);
+        pis[j] = wn[j] * vmf_inv_norm(guide_packed_kappa(pk[j]));
+    }
     int k = kGuideLobes - 1;
     float cum = 0.0f;
 #pragma unroll
@@ -230,10 +258,11 @@ LFH_DEV V3 guide_sample(const float4* __restrict__ lob, float u0, float u1, floa
         cum += pis[j];
         if (u0 < cum && k == kGuideLobes - 1) k = j;
     }
-    float4 l = lob[0];
+    uint2 sel = pk[0];
 #pragma unroll
     for (int j = 1; j < kGuideLobes; ++j)
-        if (j == k) l = lob[j * st];
+        if (j == k) sel = pk[j];
+    const float4 l = guide_unpack_lobe(sel);
     const float kappa = l.w;
     float w = 1.0f + __logf(u1 + (1.0f - u1) * __expf(-2.0f * kappa)) * __builtin_amdgcn_rcpf(kappa);
     w = fminf(fmaxf(w, -1.0f), 1.0f);

@@ -240,7 +240,10 @@ __global__ __launch_bounds__(kInferBlock, LFH_INFER_LDS_W ? 3 : 1) void k_guide_
             sum += __shfl_xor(sum, 32);
             const float inv = __builtin_amdgcn_rcpf(sum);
             const float4 lb[2] = {guide_lobe(o), guide_lobe(o + 5)};
-            const float wk0 = e0 * inv * vmf_norm(lb[0].w), wk1 = e1 * inv * vmf_norm(lb[1].w);
+            const uint2 pk0 = guide_pack_lobe(lb[0]), pk1 = guide_pack_lobe(lb[1]);
+            // the weights carry the normalisation of the stored (rounded) kappa
+            const float wk0 = e0 * inv * vmf_norm(guide_packed_kappa(pk0));
+            const float wk1 = e1 * inv * vmf_norm(guide_packed_kappa(pk1));
             float wk[2] = {wk0, wk1};
             bool ok = isfinite(inv) && isfinite(wk0) && isfinite(wk1);
 #pragma unroll
@@ -253,9 +256,8 @@ __global__ __launch_bounds__(kInferBlock, LFH_INFER_LDS_W ? 3 : 1) void k_guide_
             if (qi < n) {
                 const size_t st = QUEUE ? (size_t)q.gh.lobe_stride : 1;
                 float4* dst = lobes + (QUEUE ? (size_t)qi : (size_t)qi * kGuideLobeF4);
-                dst[(2 * grp) * st] = lb[0];
-                dst[(2 * grp + 1) * st] = lb[1];
-                reinterpret_cast<float2*>(dst + (8 + (grp >> 1)) * st)[grp & 1] = make_float2(wk[0], wk[1]);
+                *reinterpret_cast<uint4*>(dst + grp * st) = make_uint4(pk0.x, pk0.y, pk1.x, pk1.y);
+                reinterpret_cast<float2*>(dst + (4 + (grp >> 1)) * st)[grp & 1] = make_float2(wk[0], wk[1]);
             }
         };
         // The four layers of one tile from its two B fragments.

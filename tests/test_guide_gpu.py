@@ -77,8 +77,20 @@ def test_pdf_and_sampler_match_oracle():
     d, pdf = g.sample(q, seeds)
     np.testing.assert_allclose(np.linalg.norm(d, axis=1), 1.0, atol=2e-6)
     mu, kappa, pi = G.lobes(g.forward(q))
+    # the device stores each lobe's mu and kappa as binary16 (mu renormalised
+    # after the rounding); the sampler and the pdf read the same stored lobes
+    mu = mu.astype(np.float16).astype(np.float64)
+    mu /= np.linalg.norm(mu, axis=2, keepdims=True)
+    kappa = kappa.astype(np.float16).astype(np.float64)
     ref = G.mixture_pdf(mu, kappa, pi, d)
-    np.testing.assert_allclose(pdf, ref, rtol=1e-3, atol=1e-6)
+    rel = np.abs(pdf - ref) / np.maximum(ref, 1e-6)
+    # a component that the device's float32 mu rounds to the other binary16
+    # neighbour moves that lobe's term by up to kappa x 2^-11 (kappa about 7 here)
+    assert np.quantile(rel, 0.999) < 1e-3, np.quantile(rel, 0.999)
+    assert rel.max() < 2e-2, rel.max()
+    # without the rounding the pdf would differ by more than that on some samples
+    mu0, kappa0, _ = G.lobes(g.forward(q))
+    assert np.abs(G.mixture_pdf(mu0, kappa0, pi, d) - ref).max() > 0
     np.testing.assert_array_equal(g.pdf(q, d), pdf)
 
 
