@@ -89,6 +89,12 @@ constexpr int kWorkWords = kWorkTotals + 8;
 #define LFH_TRACE_SLICE 32   // C1 / C0 / C2: 16: 1557 / 1195 / 326, 24: 1557 / 1226 / 329, 32: 1573 / 1235 / 328, 48: 1558 / 1224 / 317, 96: 1457 / 1233 / 295
 #endif
 constexpr int kTraceSlice = LFH_TRACE_SLICE;  // traversal steps between refills of a wave's idle lanes
+#ifndef LFH_ANY_SLICE
+// the same for any-hit queries (C1 any hit; shadow rays of the path tracer): 32 steps 3885-3893
+// Mrays/s, 24: 4107-4131, 16: 4288-4296, 12: 4293, 8: 4257 (C2 421.5 against 428 at 12-24)
+#define LFH_ANY_SLICE 16
+#endif
+constexpr int kAnySlice = LFH_ANY_SLICE;
 constexpr int kLdsStack = LFH_LDS_STACK;  // LDS entries per lane (power of 2)
 constexpr int kTraceBlock = 256;        // threads per traversal block (4 wave64)
 constexpr uint32_t kSpillStride = 128;  // max stack entries per lane

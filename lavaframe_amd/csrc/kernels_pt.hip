@@ -305,7 +305,7 @@ __global__ __launch_bounds__(kTraceBlock, LFH_QUEUE_WAVES) void k_trace_queue(Sh
         }
         if (__ballot(active) == 0) break;
         if (active) {
-            bool done = trace_run<SHADOW, false>(ts, s, st, cull, kTraceSlice, nullptr);
+            bool done = trace_run<SHADOW, false>(ts, s, st, cull, SHADOW ? kAnySlice : kTraceSlice, nullptr);
             if (done) {
                 if (st.overflow) atomicOr(err, 1);
                 if (SHADOW) {

@@ -54,7 +54,7 @@ __global__ __launch_bounds__(kTraceBlock, ANY ? LFH_ANYHIT_WAVES : LFH_TRACE_WAV
         }
         if (__ballot(active) == 0) break;
         if (active) {
-            bool done = trace_run<ANY, STATS>(sc, s, st, cull, kTraceSlice, &stats);
+            bool done = trace_run<ANY, STATS>(sc, s, st, cull, ANY ? kAnySlice : kTraceSlice, &stats);
             if (done) {
                 if (st.overflow) atomicOr(err, 1);
                 if (ANY) {
