@@ -64,6 +64,8 @@ LFH_HD uint32_t make_inst(uint32_t inst) { return kRefInstBit | inst; }
 #define LFH_SLAB_OCTANT 1   // node box tests with the near / far planes picked by the ray octant: C1 1633 / 1590 Mrays/s (incoherent 1011 / 984), any hit 3863 / 3701, C2 398 / 392 Msamples/s
 #endif
 #ifndef LFH_ANYHIT_WAVES
+// any hit with 16-step slices: 5 / 6 / 7 / 8 waves per SIMD 4084 / 4305 / 4203 / 3860 Mrays/s
+// (7 and 8 spill 28 and 56 B per lane); closest hit at 6 waves 1642.5 against 1649 at 5
 #define LFH_ANYHIT_WAVES 6
 #endif
 #ifndef LFH_OCTANT_PACKED
