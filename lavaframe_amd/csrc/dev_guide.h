@@ -244,13 +244,9 @@ LFH_DEV void guide_frame(V3 n, V3& t, V3& b) {
 LFH_DEV V3 guide_sample(const float4* __restrict__ lob, float u0, float u1, float u2, size_t st = 1) {
     const float4 pa = lob[4 * st], pb = lob[5 * st];
     const float wn[8] = {pa.x, pa.y, pa.z, pa.w, pb.x, pb.y, pb.z, pb.w};
-    uint2 pk[8];
     float pis[8];
 #pragma unroll
-    for (int j = 0; j < kGuideLobes; ++j) {
-        pk[j] = guide_packed(lob, j, st);
-        pis[j] = wn[j] * vmf_inv_norm(guide_packed_kappa(pk[j]));
-    }
+    for (int j = 0; j < kGuideLobes; ++j) pis[j] = wn[j] * vmf_inv_norm(guide_packed_kappa(guide_packed(lob, j, st)));
     int k = kGuideLobes - 1;
     float cum = 0.0f;
 #pragma unroll
@@ -258,11 +254,8 @@ LFH_DEV V3 guide_sample(const float4* __restrict__ lob, float u0, float u1, floa
         cum += pis[j];
         if (u0 < cum && k == kGuideLobes - 1) k = j;
     }
-    uint2 sel = pk[0];
-#pragma unroll
-    for (int j = 1; j < kGuideLobes; ++j)
-        if (j == k) sel = pk[j];
-    const float4 l = guide_unpack_lobe(sel);
+    // the chosen lobe read again (the same cache lines): no eight packed lobes held in registers
+    const float4 l = guide_unpack_lobe(guide_packed(lob, k, st));
     const float kappa = l.w;
     float w = 1.0f + __logf(u1 + (1.0f - u1) * __expf(-2.0f * kappa)) * __builtin_amdgcn_rcpf(kappa);
     w = fminf(fmaxf(w, -1.0f), 1.0f);

@@ -401,7 +401,10 @@ LFH_DEV uint32_t bin_by_class(int cls, uint32_t* lds) {
     return j;
 }
 
-template <bool MULTI>
+// GUIDED: the variant for guided wavefronts (guide sampling, MIS with the
+// guide's pdf, training records); unguided wavefronts run the variant without
+// that code, which would otherwise hold registers they never use.
+template <bool MULTI, bool GUIDED>
 #ifndef LFH_SHADE_WAVES
 // k_shade register budget in waves per SIMD.  1 (153-158 VGPRs, 3 waves): C0 1207 /
 // C2 391 Msamples/s, C3 1.463 / 2.134 ms per plain / guided pass; 4 (128 VGPRs,
@@ -441,7 +444,7 @@ __global__ __launch_bounds__(256, LFH_SHADE_WAVES) void k_shade(ShadeScene sc1, 
         V3 rad = ld3(pb.rad[p]);
         V3 thr = ld3(pb.thr[p]);
         combine_nee(pb, p, rad);
-        const bool recorded = gh.train && (p % gh.stride) == gh.phase;
+        const bool recorded = GUIDED && gh.train && (p % gh.stride) == gh.phase;
         if (recorded && depth > 0) guide_rec(gh, p, depth - 1)[11] = luminance(rad);
         float4 misc = pb.misc[p];
         float prevPdf = misc.x;
@@ -517,7 +520,7 @@ __global__ __launch_bounds__(256, LFH_SHADE_WAVES) void k_shade(ShadeScene sc1, 
             const float4* lob = gh.lobes + i;
             const size_t lst = (size_t)gh.lobe_stride;
             // the strategy choice depends only on the lobes, never on a sample
-            const bool guided = gh.enabled != 0 && gh.sample != 0 && guide_valid(lob, lst);
+            const bool guided = GUIDED && gh.enabled != 0 && gh.sample != 0 && guide_valid(lob, lst);
             // next-event estimation (pathtrace.glsl:126-204)
             const V3 V = -rd;
             // the reference offsets along the unflipped normal (pathtrace.glsl:130), so
@@ -910,8 +913,12 @@ hipError_t pt_bounce(const ShadeScene& sc, int depth, int maxq, float4* rayq, ui
     }
     hipError_t e = wait_shadowed(lane, depth, s);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_shade<false>, dim3(nblk(maxq)), dim3(256), 0, s, sc, nj, depth, rayq, hitq, count, nextq,
-                       next_count, shadowq, shadow_count, pb, gh);
+    if (gh.enabled)
+        hipLaunchKernelGGL((k_shade<false, true>), dim3(nblk(maxq)), dim3(256), 0, s, sc, nj, depth, rayq, hitq, count,
+                           nextq, next_count, shadowq, shadow_count, pb, gh);
+    else
+        hipLaunchKernelGGL((k_shade<false, false>), dim3(nblk(maxq)), dim3(256), 0, s, sc, nj, depth, rayq, hitq, count,
+                           nextq, next_count, shadowq, shadow_count, pb, gh);
     if (lane)
         return shadow_on_lane<false>(sc, nj, count, hitq, shadowq, shadow_count, pb, work, err, cull, grid_blocks,
                                      *lane, s);
@@ -941,8 +948,12 @@ hipError_t pt_bounce_multi(const JobDev* jobs, int depth, int maxq, float4* rayq
     }
     hipError_t e = wait_shadowed(lane, depth, s);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_shade<true>, dim3(nblk(maxq)), dim3(256), 0, s, dummy, jobs, depth, rayq, hitq, count, nextq,
-                       next_count, shadowq, shadow_count, pb, gh);
+    if (gh.enabled)
+        hipLaunchKernelGGL((k_shade<true, true>), dim3(nblk(maxq)), dim3(256), 0, s, dummy, jobs, depth, rayq, hitq,
+                           count, nextq, next_count, shadowq, shadow_count, pb, gh);
+    else
+        hipLaunchKernelGGL((k_shade<true, false>), dim3(nblk(maxq)), dim3(256), 0, s, dummy, jobs, depth, rayq, hitq,
+                           count, nextq, next_count, shadowq, shadow_count, pb, gh);
     if (lane)
         return shadow_on_lane<true>(dummy, jobs, count, hitq, shadowq, shadow_count, pb, work, err, cull,
                                     grid_blocks, *lane, s);
