@@ -61,6 +61,8 @@ constexpr int kRecordedPaths = 1 << LFH_GUIDE_RECORD_LOG2;     // recorded paths
 // 128: 4 passes 0.173 s, 8 passes 0.175 s.  On one warm server over 32 sequences
 // (learning rate 1e-3): 2 passes 0.166 s, 4 passes 0.168, 8 passes 0.169; with
 // rate 2e-3: 2 passes 0.152 (0.157 on a second set), 3 passes 0.165, 8 passes 0.170.
+// With rate 1.5e-3, two sets of 32 sequences: 2 passes 0.152 / 0.156 s, 4 passes
+// 0.148 / 0.148 s.
 #define LFH_GUIDE_PLAIN_WARMUP 4
 #endif
 #ifndef LFH_GUIDE_LATE_REPEATS
